@@ -1,0 +1,656 @@
+// api.cpp -- libewfa host runtime: context, scratch pools, launch policy, the
+// plain per-batch and persistent launch paths, deferral passes and hipGraph
+// replay.  Replaces the reference's Sequences/Cigars host pipeline
+// (src/wavefront.cuh:43-505, src/wavefront.cu:29-330).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "ewfa_internal.h"
+#include "policy.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(EWFA_ERR_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                             \
+    } while (0)
+
+#define CHK(x)                 \
+    do {                       \
+        int r_ = (x);          \
+        if (r_ != EWFA_OK) return r_; \
+    } while (0)
+
+// Device buffer that only grows.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t n) {
+        if (n <= bytes) return EWFA_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (n == 0) return EWFA_OK;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(EWFA_ERR_NOMEM, "hipMalloc(%zu): %s", n, hipGetErrorString(e));
+        }
+        bytes = n;
+        return EWFA_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Control words in device memory.
+enum : int { CTRL_QUEUE = 0, CTRL_DEFER_A = 1, CTRL_DEFER_B = 2, CTRL_WORDS = 64 };
+
+}  // namespace
+
+struct ewfa_ctx {
+    int device = 0;
+    ewfa_opts_t opts{};
+    std::string policy_path;
+    ewfa::Policy policy;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_a0 = nullptr, ev_a1 = nullptr, ev_e1 = nullptr;
+    int num_cus = 0;
+    size_t lds_max = 0;
+    DevBuf ops, bt, ctrl, list_a, list_b;
+    uint32_t* host_ctrl = nullptr;  // pinned mirror of the control words
+    float align_ms = 0.f, emit_ms = 0.f;
+    bool timing = false;
+    // graph cache for the persistent path
+    hipGraphExec_t graph = nullptr;
+    struct Key {
+        const void *words, *pairs, *results, *cigars;
+        size_t n;
+        uint32_t max_len;
+        bool operator==(const Key& o) const {
+            return words == o.words && pairs == o.pairs && results == o.results &&
+                   cigars == o.cigars && n == o.n && max_len == o.max_len;
+        }
+    } graph_key{};
+    bool use_graph = true;
+    // host pipeline buffers
+    DevBuf h_words, h_pairs, h_results, h_cigars;
+    size_t resident_words = 0;
+};
+
+namespace {
+
+int set_device(ewfa_ctx* c) {
+    HIPCHK(hipSetDevice(c->device));
+    return EWFA_OK;
+}
+
+// Pass configuration: one launch of the align kernel.
+struct Pass {
+    int waves;
+    int dcap;
+    int bt_lds;
+    int grab;
+    int seq_words;
+    size_t lds;
+    uint32_t grid;
+    bool persistent;
+};
+
+int seq_words_for(uint32_t max_len) { return (int)(max_len / 32u) + 2; }
+
+// Dynamic LDS a pass may use: the device maximum less room for the kernel's
+// static LDS.
+size_t lds_budget(const ewfa_ctx* c) { return c->lds_max > 1024 ? c->lds_max - 256 : c->lds_max; }
+
+// Largest dcap whose slot fits `budget` bytes of LDS.
+int max_dcap_for(int seq_words, int bt_lds, size_t budget) {
+    if (ewfa::lds_bytes(seq_words, 0, bt_lds) > budget) return -1;
+    int lo = 0, hi = 1 << 16;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (ewfa::lds_bytes(seq_words, mid, bt_lds) <= budget) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+int finish_pass(ewfa_ctx* c, Pass& p, uint32_t items) {
+    p.lds = ewfa::lds_bytes(p.seq_words, p.dcap, p.bt_lds);
+    if (p.lds > lds_budget(c))
+        return fail(EWFA_ERR_RANGE, "pass needs %zu bytes of LDS (device has %zu)", p.lds, c->lds_max);
+    if (p.lds > 65536) HIPCHK(ewfa::align_set_max_lds(p.waves, p.lds));
+    if (p.persistent) {
+        int bpc = ewfa::align_blocks_per_cu(p.waves, p.lds);
+        if (bpc < 1) bpc = 1;
+        uint64_t g = (uint64_t)bpc * (uint64_t)c->num_cus;
+        if (items > 0 && g > items) g = items;
+        p.grid = (uint32_t)std::max<uint64_t>(g, 1);
+    } else {
+        p.grid = items;
+    }
+    return EWFA_OK;
+}
+
+// Main pass of a call, from the policy's bucket for max_len.
+int main_pass(ewfa_ctx* c, uint32_t max_len, uint32_t items, bool persistent, Pass& p) {
+    p = Pass{};
+    p.seq_words = seq_words_for(max_len);
+    p.persistent = persistent;
+    if (!persistent) {
+        // the reference's launch model: one workgroup of one wave per pair,
+        // a distance budget of 64 levels (src/wavefront.cuh:465)
+        p.waves = 1;
+        p.bt_lds = 1;
+        p.dcap = 64;
+        p.grab = 1;
+    } else {
+        const ewfa::BucketPolicy& b = c->policy.bucket_for(max_len);
+        p.waves = b.waves;
+        p.bt_lds = b.bt_lds;
+        p.grab = std::max(1, b.grab);
+        p.dcap = b.dcap;
+        // the bucket's dcap is a request; shrink it to what LDS holds
+        const int most = max_dcap_for(p.seq_words, p.bt_lds, lds_budget(c));
+        if (most < 0)
+            return fail(EWFA_ERR_RANGE, "sequences of %u bases do not fit in LDS", max_len);
+        if (p.dcap > most) p.dcap = most;
+        if (p.dcap < 1) p.dcap = 1;
+    }
+    return finish_pass(c, p, items);
+}
+
+// Deferral pass after one with budget prev_dcap.
+int next_pass(ewfa_ctx* c, uint32_t max_len, int prev_dcap, uint32_t items, Pass& p) {
+    p = Pass{};
+    p.seq_words = seq_words_for(max_len);
+    p.persistent = true;
+    p.grab = 1;
+    p.waves = 4;
+    const int most_global = max_dcap_for(p.seq_words, 0, lds_budget(c));
+    if (most_global < 0) return fail(EWFA_ERR_RANGE, "sequences of %u bases do not fit in LDS", max_len);
+    int want = std::max(prev_dcap * 4, 256);
+    const int dmax = (int)max_len + 1;  // no pair needs more than max(plen,tlen)
+    if (want > dmax) want = dmax;
+    if (want > most_global) want = most_global;
+    if (want <= prev_dcap) return fail(EWFA_ERR_RANGE, "no larger wavefront budget available");
+    p.dcap = want;
+    const int most_lds = max_dcap_for(p.seq_words, 1, c->lds_max / 2);
+    p.bt_lds = (most_lds >= want) ? 1 : 0;
+    if (want > 2048) p.waves = 8;
+    return finish_pass(c, p, items);
+}
+
+int reserve_scratch(ewfa_ctx* c, size_t n, uint32_t max_len) {
+    const uint32_t ops_stride = ewfa::ops_words(max_len + 1);
+    CHK(c->ops.reserve((size_t)n * ops_stride * 8));
+    CHK(c->list_a.reserve((size_t)std::max<size_t>(n, 1) * 4));
+    CHK(c->list_b.reserve((size_t)std::max<size_t>(n, 1) * 4));
+    return EWFA_OK;
+}
+
+int reserve_bt(ewfa_ctx* c, const Pass& p) {
+    if (p.bt_lds) return EWFA_OK;
+    const size_t slot = 2u * (size_t)ewfa::bt_chunks((uint32_t)p.dcap);
+    return c->bt.reserve(slot * p.grid * 8);
+}
+
+ewfa::AlignArgs base_args(ewfa_ctx* c, const uint64_t* words, const ewfa_pair_t* pairs,
+                          ewfa_result_t* results, uint32_t max_len) {
+    ewfa::AlignArgs a{};
+    a.words = words;
+    a.pairs = pairs;
+    a.results = results;
+    a.ops = c->ops.as<uint64_t>();
+    a.ops_stride = ewfa::ops_words(max_len + 1);
+    a.max_distance = c->opts.max_distance;
+    a.max_len = max_len;
+    return a;
+}
+
+void fill_pass_args(ewfa::AlignArgs& a, const Pass& p, ewfa_ctx* c) {
+    a.dcap = p.dcap;
+    a.bt_lds = p.bt_lds;
+    a.seq_words = p.seq_words;
+    a.grab = (uint32_t)p.grab;
+    a.bt_global = c->bt.as<uint64_t>();
+    a.bt_slot_words = 2u * (uint64_t)ewfa::bt_chunks((uint32_t)p.dcap);
+}
+
+// Enqueues the main pass plus emit for items [first, first+count) of the
+// identity list, deferrals collected in list_a / ctrl[CTRL_DEFER_A].
+int enqueue_main(ewfa_ctx* c, const Pass& p, const uint64_t* words, const ewfa_pair_t* pairs,
+                 size_t first, uint32_t count, uint32_t max_len, ewfa_result_t* results,
+                 int8_t* cigars, hipStream_t s, bool timing) {
+    uint32_t* ctrl = c->ctrl.as<uint32_t>();
+    ewfa::AlignArgs a = base_args(c, words, pairs, results, max_len);
+    fill_pass_args(a, p, c);
+    // batch-local view: shift the pair/result bases, the ops scratch is per pair
+    a.pairs = pairs + first;
+    a.results = results + first;
+    a.ops = c->ops.as<uint64_t>() + first * (size_t)a.ops_stride;
+    a.list = nullptr;
+    a.count = count;
+    a.count_ptr = nullptr;
+    a.queue = p.persistent ? ctrl + CTRL_QUEUE : nullptr;
+    a.defer_list = c->list_a.as<uint32_t>();
+    a.defer_count = ctrl + CTRL_DEFER_A;
+    HIPCHK(ewfa::launch_fill_u32(ctrl, 0u, 3, s));
+    if (timing) HIPCHK(hipEventRecord(c->ev_a0, s));
+    HIPCHK(ewfa::launch_align(a, p.waves, p.persistent, p.grid, p.lds, s));
+    if (timing) HIPCHK(hipEventRecord(c->ev_a1, s));
+    ewfa::EmitArgs e{};
+    e.words = words;
+    e.pairs = pairs + first;
+    e.results = results + first;
+    e.ops = a.ops;
+    e.ops_stride = a.ops_stride;
+    e.cigars = cigars;
+    e.list = nullptr;
+    e.count = count;
+    e.count_ptr = nullptr;
+    HIPCHK(ewfa::launch_emit(e, count, s));
+    if (timing) HIPCHK(hipEventRecord(c->ev_e1, s));
+    HIPCHK(hipMemcpyAsync(c->host_ctrl + CTRL_DEFER_A, ctrl + CTRL_DEFER_A, 4,
+                          hipMemcpyDeviceToHost, s));
+    return EWFA_OK;
+}
+
+// Resolves pairs deferred by the main pass of a batch: repeated passes with a
+// growing wavefront budget over the deferred list, then emit for them.
+int resolve_deferred(ewfa_ctx* c, int prev_dcap, const uint64_t* words, const ewfa_pair_t* pairs,
+                     size_t first, uint32_t max_len, ewfa_result_t* results, int8_t* cigars,
+                     hipStream_t s) {
+    uint32_t* ctrl = c->ctrl.as<uint32_t>();
+    uint32_t pending = c->host_ctrl[CTRL_DEFER_A];
+    uint32_t* in_list = c->list_a.as<uint32_t>();
+    uint32_t* out_list = c->list_b.as<uint32_t>();
+    int in_ctr = CTRL_DEFER_A, out_ctr = CTRL_DEFER_B;
+    int dcap = prev_dcap;
+    while (pending > 0) {
+        Pass p;
+        int rc = next_pass(c, max_len, dcap, pending, p);
+        if (rc != EWFA_OK) {
+            // no budget left: these pairs are beyond what the device can stage
+            std::vector<uint32_t> ids(pending);
+            HIPCHK(hipMemcpyAsync(ids.data(), in_list, pending * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (uint32_t id : ids) {
+                ewfa_result_t r{0, 0, EWFA_PAIR_TOO_DISTANT, 0};
+                HIPCHK(hipMemcpyAsync(results + first + id, &r, sizeof r, hipMemcpyHostToDevice, s));
+                HIPCHK(hipStreamSynchronize(s));
+            }
+            g_err.clear();
+            return EWFA_OK;
+        }
+        CHK(reserve_bt(c, p));
+        ewfa::AlignArgs a = base_args(c, words, pairs, results, max_len);
+        fill_pass_args(a, p, c);
+        a.pairs = pairs + first;
+        a.results = results + first;
+        a.ops = c->ops.as<uint64_t>() + first * (size_t)a.ops_stride;
+        a.list = in_list;
+        a.count = pending;
+        a.count_ptr = nullptr;
+        a.queue = ctrl + CTRL_QUEUE;
+        a.defer_list = out_list;
+        a.defer_count = ctrl + out_ctr;
+        HIPCHK(ewfa::launch_fill_u32(ctrl + CTRL_QUEUE, 0u, 1, s));
+        HIPCHK(ewfa::launch_fill_u32(ctrl + out_ctr, 0u, 1, s));
+        HIPCHK(ewfa::launch_align(a, p.waves, true, p.grid, p.lds, s));
+        ewfa::EmitArgs e{};
+        e.words = words;
+        e.pairs = pairs + first;
+        e.results = results + first;
+        e.ops = a.ops;
+        e.ops_stride = a.ops_stride;
+        e.cigars = cigars;
+        e.list = in_list;
+        e.count = pending;
+        HIPCHK(ewfa::launch_emit(e, pending, s));
+        HIPCHK(hipMemcpyAsync(c->host_ctrl + out_ctr, ctrl + out_ctr, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        pending = c->host_ctrl[out_ctr];
+        std::swap(in_list, out_list);
+        std::swap(in_ctr, out_ctr);
+        dcap = p.dcap;
+    }
+    return EWFA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ewfa_default_opts(ewfa_opts_t* o) {
+    if (!o) return;
+    memset(o, 0, sizeof *o);
+    o->device = 0;
+    o->mode = EWFA_MODE_PERSISTENT;
+    o->max_distance = 0;
+    o->batch_size = 0;
+    o->policy_path = nullptr;
+}
+
+const char* ewfa_last_error(void) { return g_err.c_str(); }
+
+int ewfa_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int ewfa_device_info(int device, char* name, int name_cap, int* major, int* minor) {
+    int n = ewfa_device_count();
+    if (n <= 0) return fail(EWFA_ERR_NODEV, "no HIP device");
+    if (device < 0 || device >= n) return fail(EWFA_ERR_ARG, "device %d out of range", device);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (name && name_cap > 0) snprintf(name, (size_t)name_cap, "%s", prop.name);
+    if (major) *major = prop.major;
+    if (minor) *minor = prop.minor;
+    return EWFA_OK;
+}
+
+int ewfa_create(ewfa_ctx_t** out, const ewfa_opts_t* opts) {
+    if (!out) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    *out = nullptr;
+    ewfa_opts_t o;
+    ewfa_default_opts(&o);
+    if (opts) o = *opts;
+    if (o.mode != EWFA_MODE_PLAIN && o.mode != EWFA_MODE_PERSISTENT)
+        return fail(EWFA_ERR_ARG, "unknown mode %d", o.mode);
+    if (o.max_distance < 0 || o.batch_size < 0) return fail(EWFA_ERR_ARG, "negative option");
+    int n = ewfa_device_count();
+    if (n <= 0) return fail(EWFA_ERR_NODEV, "no HIP device");
+    if (o.device < 0 || o.device >= n) return fail(EWFA_ERR_ARG, "device %d out of range", o.device);
+    ewfa_ctx* c = new ewfa_ctx();
+    c->device = o.device;
+    c->opts = o;
+    if (o.policy_path) {
+        c->policy_path = o.policy_path;
+        c->opts.policy_path = c->policy_path.c_str();
+    }
+    int rc = set_device(c);
+    if (rc == EWFA_OK) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) rc = fail(EWFA_ERR_HIP, "props");
+        else {
+            c->num_cus = prop.multiProcessorCount;
+            // per-block LDS a kernel may opt into (160 KiB on gfx950)
+            c->lds_max = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
+            if (prop.maxSharedMemoryPerMultiProcessor > 0)
+                c->lds_max = std::min(c->lds_max, (size_t)prop.maxSharedMemoryPerMultiProcessor);
+            if (c->lds_max == 0) c->lds_max = 65536;
+        }
+    }
+    if (rc == EWFA_OK) rc = ewfa::load_policy(c->opts.policy_path, c->policy);
+    if (rc == EWFA_OK && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(EWFA_ERR_HIP, "hipStreamCreate failed");
+    if (rc == EWFA_OK) {
+        if (hipEventCreate(&c->ev_in) != hipSuccess || hipEventCreate(&c->ev_a0) != hipSuccess ||
+            hipEventCreate(&c->ev_a1) != hipSuccess || hipEventCreate(&c->ev_e1) != hipSuccess)
+            rc = fail(EWFA_ERR_HIP, "hipEventCreate failed");
+    }
+    if (rc == EWFA_OK) rc = c->ctrl.reserve(CTRL_WORDS * 4);
+    if (rc == EWFA_OK && hipHostMalloc((void**)&c->host_ctrl, CTRL_WORDS * 4, hipHostMallocDefault) != hipSuccess)
+        rc = fail(EWFA_ERR_NOMEM, "hipHostMalloc failed");
+    if (rc != EWFA_OK) {
+        ewfa_destroy(c);
+        return rc;
+    }
+    memset(c->host_ctrl, 0, CTRL_WORDS * 4);
+    *out = c;
+    return EWFA_OK;
+}
+
+int ewfa_destroy(ewfa_ctx_t* c) {
+    if (!c) return EWFA_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    for (hipEvent_t e : {c->ev_in, c->ev_a0, c->ev_a1, c->ev_e1})
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    for (DevBuf* b : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->h_words, &c->h_pairs,
+                      &c->h_results, &c->h_cigars})
+        b->release();
+    if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
+    delete c;
+    return EWFA_OK;
+}
+
+int ewfa_device_alloc(int device, size_t bytes, void** ptr) {
+    if (!ptr) return fail(EWFA_ERR_ARG, "ptr is NULL");
+    *ptr = nullptr;
+    if (ewfa_device_count() <= 0) return fail(EWFA_ERR_NODEV, "no HIP device");
+    HIPCHK(hipSetDevice(device));
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 4);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(EWFA_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return EWFA_OK;
+}
+
+int ewfa_device_free(void* ptr) {
+    if (ptr) HIPCHK(hipFree(ptr));
+    return EWFA_OK;
+}
+
+int ewfa_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+    if (bytes == 0) return EWFA_OK;
+    if (!dst || !src) return fail(EWFA_ERR_ARG, "null pointer");
+    hipMemcpyKind k;
+    switch (kind) {
+        case 0: k = hipMemcpyHostToDevice; break;
+        case 1: k = hipMemcpyDeviceToHost; break;
+        case 2: k = hipMemcpyDeviceToDevice; break;
+        default: return fail(EWFA_ERR_ARG, "unknown copy kind %d", kind);
+    }
+    HIPCHK(hipMemcpy(dst, src, bytes, k));
+    return EWFA_OK;
+}
+
+int ewfa_device_synchronize(int device) {
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipDeviceSynchronize());
+    return EWFA_OK;
+}
+
+int ewfa_pack_ascii(const char* d_ascii, size_t ascii_stride, const uint32_t* d_len, size_t nseq,
+                    uint64_t* d_words, size_t words_stride, void* stream) {
+    if (nseq == 0) return EWFA_OK;
+    if (!d_ascii || !d_len || !d_words || words_stride == 0)
+        return fail(EWFA_ERR_ARG, "null buffer or zero stride");
+    if (words_stride < ewfa_packed_words(ascii_stride) && words_stride * 32 < ascii_stride)
+        return fail(EWFA_ERR_ARG, "words_stride too small for ascii_stride");
+    HIPCHK(ewfa::launch_pack(d_ascii, ascii_stride, d_len, nseq, d_words, words_stride,
+                             (hipStream_t)stream));
+    return EWFA_OK;
+}
+
+int ewfa_max_distance(ewfa_ctx_t* c, uint32_t max_len) {
+    if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    int most = max_dcap_for(seq_words_for(max_len), 0, lds_budget(c));
+    if (most < 0) return fail(EWFA_ERR_RANGE, "sequences of %u bases do not fit in LDS", max_len);
+    return std::min(most, (int)max_len + 1);
+}
+
+int ewfa_set_flags(ewfa_ctx_t* c, int flags) {
+    if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    if (flags & ~(EWFA_FLAG_TIMING | EWFA_FLAG_NO_GRAPH)) return fail(EWFA_ERR_ARG, "unknown flag");
+    c->timing = (flags & EWFA_FLAG_TIMING) != 0;
+    c->use_graph = (flags & EWFA_FLAG_NO_GRAPH) == 0;
+    return EWFA_OK;
+}
+
+int ewfa_last_timing(ewfa_ctx_t* c, float* align_ms, float* emit_ms) {
+    if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    if (align_ms) *align_ms = c->align_ms;
+    if (emit_ms) *emit_ms = c->emit_ms;
+    return EWFA_OK;
+}
+
+int ewfa_align_device(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t* d_pairs, size_t n,
+                      uint32_t max_len, ewfa_result_t* d_results, int8_t* d_cigars, void* stream) {
+    if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    if (n == 0) return EWFA_OK;
+    if (!d_words || !d_pairs || !d_results || !d_cigars) return fail(EWFA_ERR_ARG, "null buffer");
+    if (n > 0xFFFFFFF0u) return fail(EWFA_ERR_RANGE, "too many pairs in one call");
+    CHK(set_device(c));
+    hipStream_t s = c->stream;
+    // order the work after what the caller queued on its stream
+    HIPCHK(hipEventRecord(c->ev_in, (hipStream_t)stream));
+    HIPCHK(hipStreamWaitEvent(s, c->ev_in, 0));
+    CHK(reserve_scratch(c, n, max_len));
+    c->align_ms = c->emit_ms = 0.f;
+
+    if (c->opts.mode == EWFA_MODE_PERSISTENT) {
+        Pass p;
+        CHK(main_pass(c, max_len, (uint32_t)n, true, p));
+        CHK(reserve_bt(c, p));
+        ewfa_ctx::Key key{d_words, d_pairs, d_results, d_cigars, n, max_len};
+        const bool timing = c->timing;
+        if (c->use_graph && !timing) {
+            if (!(c->graph && c->graph_key == key)) {
+                if (c->graph) {
+                    (void)hipGraphExecDestroy(c->graph);
+                    c->graph = nullptr;
+                }
+                hipGraph_t g = nullptr;
+                HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                int rc = enqueue_main(c, p, d_words, d_pairs, 0, (uint32_t)n, max_len, d_results,
+                                      d_cigars, s, false);
+                hipError_t ce = hipStreamEndCapture(s, &g);
+                if (rc != EWFA_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (ce != hipSuccess) return fail(EWFA_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+                hipError_t ie = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ie != hipSuccess) {
+                    c->graph = nullptr;
+                    return fail(EWFA_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ie));
+                }
+                c->graph_key = key;
+            }
+            HIPCHK(hipEventRecord(c->ev_a0, s));
+            HIPCHK(hipGraphLaunch(c->graph, s));
+            HIPCHK(hipEventRecord(c->ev_e1, s));
+            HIPCHK(hipStreamSynchronize(s));
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, c->ev_a0, c->ev_e1) == hipSuccess) c->align_ms = ms;
+        } else {
+            CHK(enqueue_main(c, p, d_words, d_pairs, 0, (uint32_t)n, max_len, d_results, d_cigars,
+                             s, timing));
+            HIPCHK(hipStreamSynchronize(s));
+            if (timing) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, c->ev_a0, c->ev_a1) == hipSuccess) c->align_ms = ms;
+                if (hipEventElapsedTime(&ms, c->ev_a1, c->ev_e1) == hipSuccess) c->emit_ms = ms;
+            }
+        }
+        CHK(resolve_deferred(c, p.dcap, d_words, d_pairs, 0, max_len, d_results, d_cigars, s));
+        return EWFA_OK;
+    }
+
+    // plain: the reference's per-batch launch model (src/aligner.cu:62-66)
+    const size_t batch = c->opts.batch_size > 0 ? (size_t)c->opts.batch_size : n;
+    for (size_t first = 0; first < n; first += batch) {
+        const uint32_t count = (uint32_t)std::min(batch, n - first);
+        Pass p;
+        CHK(main_pass(c, max_len, count, false, p));
+        CHK(enqueue_main(c, p, d_words, d_pairs, first, count, max_len, d_results, d_cigars, s,
+                         c->timing));
+        HIPCHK(hipStreamSynchronize(s));
+        if (c->timing) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, c->ev_a0, c->ev_a1) == hipSuccess) c->align_ms += ms;
+            if (hipEventElapsedTime(&ms, c->ev_a1, c->ev_e1) == hipSuccess) c->emit_ms += ms;
+        }
+        CHK(resolve_deferred(c, p.dcap, d_words, d_pairs, first, max_len, d_results, d_cigars, s));
+    }
+    return EWFA_OK;
+}
+
+int ewfa_upload_words(ewfa_ctx_t* c, const uint64_t* h_words, size_t n_words) {
+    if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    if (!h_words && n_words) return fail(EWFA_ERR_ARG, "null buffer");
+    CHK(set_device(c));
+    CHK(c->h_words.reserve(std::max<size_t>(n_words, 1) * 8));
+    if (n_words) {
+        HIPCHK(hipMemcpyAsync(c->h_words.p, h_words, n_words * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    c->resident_words = n_words;
+    return EWFA_OK;
+}
+
+int ewfa_align_resident(ewfa_ctx_t* c, const ewfa_pair_t* h_pairs, size_t n, ewfa_result_t* h_results,
+                        int8_t* h_cigars, size_t cigar_bytes) {
+    if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    if (n == 0) return EWFA_OK;
+    if (!h_pairs || !h_results || !h_cigars) return fail(EWFA_ERR_ARG, "null buffer");
+    if (!c->h_words.p) return fail(EWFA_ERR_STATE, "no words uploaded");
+    const size_t n_words = c->resident_words;
+    uint32_t max_len = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const ewfa_pair_t& p = h_pairs[i];
+        max_len = std::max(max_len, std::max(p.plen, p.tlen));
+        if (p.pattern_off + ewfa_packed_words(p.plen) - 1 > n_words ||
+            p.text_off + ewfa_packed_words(p.tlen) - 1 > n_words)
+            return fail(EWFA_ERR_ARG, "pair %zu reads past the packed words", i);
+        if ((p.cigar_off & 3) || (p.cigar_cap & 3) || p.cigar_off + p.cigar_cap > cigar_bytes)
+            return fail(EWFA_ERR_ARG, "pair %zu has a bad CIGAR region", i);
+    }
+    CHK(set_device(c));
+    CHK(c->h_pairs.reserve(n * sizeof(ewfa_pair_t)));
+    CHK(c->h_results.reserve(n * sizeof(ewfa_result_t)));
+    CHK(c->h_cigars.reserve(std::max<size_t>(cigar_bytes, 4)));
+    hipStream_t s = c->stream;
+    HIPCHK(hipMemcpyAsync(c->h_pairs.p, h_pairs, n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, s));
+    CHK(ewfa_align_device(c, c->h_words.as<uint64_t>(), c->h_pairs.as<ewfa_pair_t>(), n, max_len,
+                          c->h_results.as<ewfa_result_t>(), c->h_cigars.as<int8_t>(), s));
+    HIPCHK(hipMemcpyAsync(h_results, c->h_results.p, n * sizeof(ewfa_result_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h_cigars, c->h_cigars.p, cigar_bytes, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return EWFA_OK;
+}
+
+int ewfa_align_host(ewfa_ctx_t* c, const uint64_t* h_words, size_t n_words, const ewfa_pair_t* h_pairs,
+                    size_t n, ewfa_result_t* h_results, int8_t* h_cigars, size_t cigar_bytes) {
+    if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    if (n == 0) return EWFA_OK;
+    if (!h_words) return fail(EWFA_ERR_ARG, "null buffer");
+    CHK(ewfa_upload_words(c, h_words, n_words));
+    return ewfa_align_resident(c, h_pairs, n, h_results, h_cigars, cigar_bytes);
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// Internal declarations shared by the HIP kernels (kernels.hip) and the host
+// runtime (api.cpp, serve.cpp, tune.cpp).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ewfa.h"
+
+namespace ewfa {
+
+// Backtrace op codes of the reference (src/wavefront_structures.h:66-71).
+enum : int { OP_NOOP = 0, OP_INS = 1, OP_SUB = 2, OP_DEL = 3 };
+
+// Per-pair status values used only between the two kernels.
+constexpr int32_t PAIR_DEFERRED = 100;  // distance beyond this pass's budget
+
+// Backtrace storage: one "chunk" per (level d, 64 consecutive diagonals)
+// holding two 64-bit ballots (op bit 0 and op bit 1).  Level d >= 1 has
+// d/32 + 1 chunks (2d+1 diagonals); chunks of levels 1..d-1 come first.
+__host__ __device__ inline uint32_t bt_level_base(uint32_t d) {
+    const uint32_t q = d >> 5, r = d & 31u;
+    return d + 16u * q * (q - (q ? 1u : 0u)) + q * r - 1u;
+}
+// Chunks needed for levels 1..dmax.
+__host__ __device__ inline uint32_t bt_chunks(uint32_t dmax) { return bt_level_base(dmax + 1); }
+
+// Words of the ops scratch per pair: op s (0-based) at bits 2(s%32) of word s/32.
+__host__ __device__ inline uint32_t ops_words(uint32_t dmax) { return dmax / 32u + 1u; }
+
+// How one length bucket of pairs is processed.
+struct BucketPolicy {
+    uint32_t len_hi;     // pairs with max(plen,tlen) <= len_hi (inclusive)
+    int32_t waves;       // waves per pair: 1 = wave-per-pair, else block-per-pair
+    int32_t dcap;        // wavefront levels held by this pass (deferred beyond)
+    int32_t bt_lds;      // 1: backtrace in LDS, 0: in the HBM slot region
+    int32_t grab;        // pairs taken per queue pull
+};
+
+constexpr int MAX_BUCKETS = 8;
+
+// Arguments of the align kernel (passed by value).
+struct AlignArgs {
+    const uint64_t* words;
+    const ewfa_pair_t* pairs;
+    ewfa_result_t* results;
+    uint64_t* ops;            // ops scratch, ops_stride words per pair
+    uint32_t ops_stride;
+    // work list: list[0..count) pair indices, count read from *count_ptr when
+    // count_ptr != null, else `count`
+    const uint32_t* list;     // null = identity
+    uint32_t count;
+    const uint32_t* count_ptr;
+    uint32_t* queue;          // persistent: queue head counter (null = plain)
+    uint32_t grab;
+    // deferral output (pairs whose distance exceeds dcap)
+    uint32_t* defer_list;
+    uint32_t* defer_count;
+    // per-block HBM backtrace slot region (bt_lds == 0)
+    uint64_t* bt_global;
+    uint64_t bt_slot_words;
+    int32_t dcap;
+    int32_t max_distance;     // reference cap, 0 = none
+    int32_t seq_words;        // LDS words per sequence (pattern, text)
+    int32_t bt_lds;
+    uint32_t max_len;
+};
+
+struct EmitArgs {
+    const uint64_t* words;
+    const ewfa_pair_t* pairs;
+    ewfa_result_t* results;
+    const uint64_t* ops;
+    uint32_t ops_stride;
+    int8_t* cigars;
+    // pairs to emit: list[0..count) (list null = identity), count read from
+    // *count_ptr when count_ptr != null
+    const uint32_t* list;
+    uint32_t count;
+    const uint32_t* count_ptr;
+};
+
+// LDS bytes one pair slot needs for the given budget.
+inline size_t lds_bytes(int seq_words, int dcap, int bt_lds) {
+    size_t b = 16u * (size_t)seq_words;             // pattern + text
+    b += 8u * (size_t)(2 * dcap + 5);                // two offset generations
+    b = (b + 15) & ~(size_t)15;
+    if (bt_lds) b += 16u * (size_t)bt_chunks((uint32_t)dcap);
+    return b;
+}
+
+// Kernel launchers (kernels.hip).  All asynchronous on `stream`.
+hipError_t launch_align(const AlignArgs& a, int waves, bool persistent, uint32_t grid,
+                        size_t lds, hipStream_t stream);
+hipError_t launch_emit(const EmitArgs& a, uint32_t grid_items, hipStream_t stream);
+hipError_t launch_pack(const char* ascii, size_t ascii_stride, const uint32_t* len, size_t nseq,
+                       uint64_t* words, size_t words_stride, hipStream_t stream);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
+// Occupancy of the align kernel for a configuration (blocks per CU).
+int align_blocks_per_cu(int waves, size_t lds);
+hipError_t align_set_max_lds(int waves, size_t lds);
+
+}  // namespace ewfa

@@ -1,0 +1,64 @@
+// policy.cpp -- built-in launch policy and policy files.
+#include "policy.h"
+
+#include <errno.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+
+namespace ewfa {
+
+const BucketPolicy& Policy::bucket_for(uint32_t len) const {
+    for (const BucketPolicy& b : buckets)
+        if (len <= b.len_hi) return b;
+    return buckets.back();
+}
+
+Policy default_policy() {
+    Policy p;
+    //              len_hi   waves dcap  bt_lds grab
+    p.buckets.push_back({512u, 1, 64, 1, 16});
+    p.buckets.push_back({2048u, 1, 96, 1, 8});
+    p.buckets.push_back({8192u, 4, 512, 0, 1});
+    p.buckets.push_back({0xFFFFFFFFu, 8, 2048, 0, 1});
+    return p;
+}
+
+int load_policy(const char* path, Policy& out) {
+    out = default_policy();
+    if (!path || !*path) return EWFA_OK;
+    FILE* f = fopen(path, "r");
+    if (!f) return EWFA_ERR_IO;
+    Policy p;
+    char line[512];
+    while (fgets(line, sizeof line, f)) {
+        char* hash = strchr(line, '#');
+        if (hash) *hash = 0;
+        unsigned long len_hi;
+        int waves, dcap, bt, grab;
+        if (sscanf(line, "%lu %d %d %d %d", &len_hi, &waves, &dcap, &bt, &grab) != 5) continue;
+        if (waves != 1 && waves != 2 && waves != 4 && waves != 8 && waves != 16) continue;
+        if (dcap < 1 || grab < 1) continue;
+        p.buckets.push_back({(uint32_t)(len_hi > 0xFFFFFFFFul ? 0xFFFFFFFFul : len_hi), waves, dcap,
+                             bt ? 1 : 0, grab});
+    }
+    fclose(f);
+    if (p.buckets.empty()) return EWFA_ERR_FORMAT;
+    p.buckets.back().len_hi = 0xFFFFFFFFu;  // the last bucket takes every longer pair
+    out = p;
+    return EWFA_OK;
+}
+
+int save_policy(const char* path, const Policy& p) {
+    FILE* f = fopen(path, "w");
+    if (!f) return EWFA_ERR_IO;
+    fprintf(f, "# len_hi waves dcap bt_lds grab\n");
+    for (const BucketPolicy& b : p.buckets)
+        fprintf(f, "%u %d %d %d %d\n", b.len_hi, b.waves, b.dcap, b.bt_lds, b.grab);
+    fclose(f);
+    return EWFA_OK;
+}
+
+}  // namespace ewfa

@@ -1,0 +1,210 @@
+"""Parity of the HIP path (through the C ABI) with the oracle, bit for bit.
+
+Every case prints its name and sizes before touching the GPU.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import ewfa
+import oracle_py as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_host(words, pairs, n, cigar_bytes, mode=ewfa.MODE_PERSISTENT, max_distance=0, flags=0, batch_size=0):
+    with ewfa.Context(mode=mode, max_distance=max_distance, flags=flags, batch_size=batch_size) as ctx:
+        res, cig = ctx.align_host(words, pairs, n, cigar_bytes)
+    return res, cig
+
+
+def assert_matches_oracle(res, pairs, cig, o_dist, o_streams, label, max_distance=0):
+    streams = ewfa.streams_from(res, pairs, cig)
+    bad = []
+    for i in range(len(res)):
+        st = int(res[i]["status"])
+        want_status = (ewfa.PAIR_OK,) if max_distance == 0 else (ewfa.PAIR_OK, ewfa.PAIR_CAPPED,
+                                                                 ewfa.PAIR_UNDEFINED)
+        if st not in want_status or int(res[i]["distance"]) != int(o_dist[i]) or \
+                not np.array_equal(streams[i], o_streams[i]):
+            bad.append(i)
+            if len(bad) > 5:
+                break
+    if bad:
+        i = bad[0]
+        raise AssertionError(
+            f"{label}: {len(bad)}+ mismatching pairs; first {i}: status {int(res[i]['status'])} "
+            f"dist {int(res[i]['distance'])} vs {int(o_dist[i])}; "
+            f"gpu {streams[i][:40].tolist()} vs oracle {o_streams[i][:40].tolist()}")
+
+
+def workload_case(name, n, first=0, **over):
+    spec = dict(ewfa.CONFIGS[name])
+    spec.update(over)
+    return ewfa.Workload(ewfa.synth_spec(**spec), n, first=first)
+
+
+@pytest.mark.parametrize("mode", [ewfa.MODE_PERSISTENT, ewfa.MODE_PLAIN])
+def test_c1_all_pairs(mode):
+    say(f"case c1 10000x150bp mode={mode}")
+    w = workload_case("c1", 10_000)
+    ok, d, sl, st = orc.batch_workload(w)
+    assert ok == w.n
+    res, cig = run_host(w.words, w.pairs, w.n, w.cigar_bytes, mode=mode, batch_size=3000)
+    assert_matches_oracle(res, w.pairs, cig, d, st, "c1")
+
+
+@pytest.mark.parametrize("mode", [ewfa.MODE_PERSISTENT, ewfa.MODE_PLAIN])
+def test_c2_subset(mode):
+    say(f"case c2 3000x1000bp (pairs 500000..) mode={mode}")
+    w = workload_case("c2", 3000, first=500_000)
+    ok, d, sl, st = orc.batch_workload(w)
+    assert ok == w.n
+    res, cig = run_host(w.words, w.pairs, w.n, w.cigar_bytes, mode=mode, batch_size=1000)
+    assert_matches_oracle(res, w.pairs, cig, d, st, "c2")
+    # distance is also the Levenshtein distance
+    for i in range(0, w.n, 300):
+        assert int(res[i]["distance"]) == orc.levenshtein(w.pattern(i), w.text(i))
+
+
+def test_c2_reference_stride():
+    """The reference's fixed per-pair CIGAR buffer (1004 bytes at -l 1000)."""
+    say("case c2 reference stride 1004, 2000 pairs")
+    spec = ewfa.synth_spec(**ewfa.CONFIGS["c2"])
+    w = ewfa.Workload(spec, 2000, first=7, cigar_cap=1004)
+    ok, d, sl, st = orc.batch_workload(w)
+    res, cig = run_host(w.words, w.pairs, w.n, w.cigar_bytes)
+    assert_matches_oracle(res, w.pairs, cig, d, st, "c2-stride")
+
+
+def test_c3_subset_mixed_lengths():
+    say("case c3 1500 pairs log-uniform 100..10000 bp")
+    w = workload_case("c3", 1500, first=12345)
+    ok, d, sl, st = orc.batch_workload(w)
+    assert ok == w.n
+    res, cig = run_host(w.words, w.pairs, w.n, w.cigar_bytes)
+    assert_matches_oracle(res, w.pairs, cig, d, st, "c3")
+    assert int(d.max()) > 64  # the deferral passes were exercised
+
+
+def test_c4_kind_short():
+    say("case c4-kind 6 pairs 6000..8000 bp 10-15% with indel runs")
+    w = ewfa.Workload(ewfa.synth_spec(kind=2, seed=4, len_lo=6000, len_hi=8000, err_lo=0.10, err_hi=0.15), 6)
+    ok, d, sl, st = orc.batch_workload(w)
+    assert ok == w.n
+    res, cig = run_host(w.words, w.pairs, w.n, w.cigar_bytes)
+    assert_matches_oracle(res, w.pairs, cig, d, st, "c4-kind")
+
+
+EDGE = [
+    (b"", b""), (b"A", b""), (b"", b"A"), (b"ACGT", b"ACGT"), (b"AA", b"A"), (b"ACGT", b"AGGT"),
+    (b"AC", b"CA"), (b"A" * 300, b"A" * 300), (b"A" * 300, b"C" * 300), (b"A" * 200, b"A" * 450),
+    (b"N" * 40, b"G" * 40), (b"acgtu" * 30, b"ACGTT" * 30), (b"ACGT" * 16, b"ACGT" * 8),
+    (b"A" * 32, b"A" * 33), (b"A" * 64, b"T" + b"A" * 64), (b"G" * 1, b"C" * 1),
+    (b"ACGTACGTAC" * 13, b"ACGTACGTAC" * 13 + b"G"), (b"T" * 127, b"T" * 127), (b"T" * 254, b"T" * 254),
+    (b"CA" * 70, b"AC" * 70), (b"A" * 10 + b"C" * 10, b"C" * 10 + b"A" * 10),
+]
+
+
+def test_edge_cases():
+    say(f"case edge {len(EDGE)} hand pairs")
+    P = [p for p, _ in EDGE]
+    T = [t for _, t in EDGE]
+    words, pairs, cb, ml = ewfa.from_sequences(P, T)
+    res, cig = run_host(words, pairs, len(P), cb)
+    streams = ewfa.streams_from(res, pairs, cig)
+    for i, (p, t) in enumerate(EDGE):
+        d, s = orc.stream(p, t)
+        assert int(res[i]["status"]) == ewfa.PAIR_OK, (p, t)
+        assert int(res[i]["distance"]) == d, (p, t)
+        assert streams[i].tolist() == s, (p, t, streams[i].tolist(), s)
+        assert ewfa.cigar_text(streams[i]) == orc.text(s)
+
+
+def test_reference_cap_64():
+    """max_distance=64 reproduces the reference's capped output."""
+    say("case cap64 c3-kind high error 400 pairs")
+    w = ewfa.Workload(ewfa.synth_spec(kind=1, seed=9, len_lo=300, len_hi=3000, err_lo=0.02, err_hi=0.10), 400)
+    ok, d, sl, st = orc.batch_workload(w, max_distance=64)
+    res, cig = run_host(w.words, w.pairs, w.n, w.cigar_bytes, max_distance=64)
+    assert_matches_oracle(res, w.pairs, cig, d, st, "cap64", max_distance=64)
+    capped = res["status"] == ewfa.PAIR_CAPPED
+    assert capped.sum() > 10
+    assert (res["distance"][capped] == 64).all()
+
+
+def test_plain_equals_persistent_and_graph_replays():
+    say("case replay c2 2000 pairs: plain, eager persistent, graph x3")
+    w = workload_case("c2", 2000, first=99)
+    ref, refc = run_host(w.words, w.pairs, w.n, w.cigar_bytes, mode=ewfa.MODE_PLAIN, batch_size=700)
+    with ewfa.Context(mode=ewfa.MODE_PERSISTENT, flags=ewfa.FLAG_NO_GRAPH) as ctx:
+        r1, c1 = ctx.align_host(w.words, w.pairs, w.n, w.cigar_bytes)
+    assert np.array_equal(ref, r1) and np.array_equal(refc, c1)
+    with ewfa.Context(mode=ewfa.MODE_PERSISTENT) as ctx:
+        for rep in range(3):
+            r2, c2 = ctx.align_host(w.words, w.pairs, w.n, w.cigar_bytes)
+            assert np.array_equal(ref, r2) and np.array_equal(refc, c2), rep
+
+
+def test_device_api_with_library_buffers():
+    say("case device api c2 1000 pairs, device buffers, timing flag")
+    w = workload_case("c2", 1000, first=3)
+    ok, d, sl, st = orc.batch_workload(w)
+    dw = ewfa.DeviceBuffer.from_array(w.words)
+    dp = ewfa.DeviceBuffer.from_array(w.pairs[:w.n])
+    dr = ewfa.DeviceBuffer(w.n * 16)
+    dc = ewfa.DeviceBuffer(w.cigar_bytes)
+    with ewfa.Context(flags=ewfa.FLAG_TIMING) as ctx:
+        ctx.align_device(dw.ptr, dp.ptr, w.n, w.max_len, dr.ptr, dc.ptr, None)
+        a_ms, e_ms = ctx.last_timing()
+    ewfa.synchronize()
+    assert a_ms > 0 and e_ms > 0
+    res = dr.download(ewfa.RESULT_DTYPE, w.n)
+    cig = dc.download(np.int8)
+    assert_matches_oracle(res, w.pairs, cig, d, st, "device-api")
+
+
+def test_pack_kernel_matches_host():
+    say("case pack kernel 500 sequences, stride 304")
+    rng = np.random.default_rng(5)
+    stride = 304
+    n = 500
+    lens = rng.integers(0, stride + 1, n).astype(np.uint32)
+    ascii = rng.choice(np.frombuffer(b"ACGTNacgtU", dtype=np.uint8), size=n * stride).astype(np.uint8)
+    wstride = int(ewfa.lib().ewfa_packed_words(stride))
+    expect = np.zeros(n * wstride, dtype=np.uint64)
+    for j in range(n):
+        seq = ascii[j * stride: j * stride + int(lens[j])].tobytes()
+        ew = orc.pack_u64(seq)
+        expect[j * wstride: j * wstride + len(ew)] = ew
+    da = ewfa.DeviceBuffer.from_array(ascii)
+    dl = ewfa.DeviceBuffer.from_array(lens)
+    dw = ewfa.DeviceBuffer.from_array(np.full(n * wstride, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64))
+    rc = ewfa.lib().ewfa_pack_ascii(da.ptr, stride, dl.ptr, n, dw.ptr, wstride, None)
+    assert rc == 0
+    ewfa.synchronize()
+    got = dw.download(np.uint64)
+    assert np.array_equal(got, expect)
+
+
+def test_cigar_overflow_status():
+    say("case overflow cap 8 bytes, 50 c1 pairs")
+    w = workload_case("c1", 50)
+    w.layout(cigar_cap=8)
+    ok, d, sl, st = orc.batch_workload(w)
+    res, cig = run_host(w.words, w.pairs, w.n, w.cigar_bytes)
+    for i in range(w.n):
+        full = len(st[i])
+        if full > 8:
+            assert int(res[i]["status"]) == ewfa.PAIR_OVERFLOW
+            assert int(res[i]["cigar_len"]) == full
+            assert cig[8 * i: 8 * i + 8].tolist() == st[i][:8].tolist()
+        else:
+            assert int(res[i]["status"]) == ewfa.PAIR_OK
+            assert cig[8 * i: 8 * i + full].tolist() == st[i].tolist()
