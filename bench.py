@@ -41,8 +41,8 @@ def main():
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--batch", type=int, default=100_000)
-    ap.add_argument("--cpu-sample", type=int, default=200_000)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=3000, help="pairs checked against the oracle")
     ap.add_argument("--profile-run", action="store_true",
@@ -172,7 +172,7 @@ def main():
         cores = threads
         done = 0
         t0 = time.perf_counter()
-        chunk = 20_000
+        chunk = 100_000
         while done < args.cpu_sample and time.perf_counter() - t0 < args.cpu_seconds:
             idx = np.arange(done, min(done + chunk, args.cpu_sample, n))
             orc.batch_workload(w, idx=idx, threads=cores)

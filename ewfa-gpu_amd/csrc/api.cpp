@@ -118,12 +118,17 @@ struct Pass {
     int bt_lds;
     int grab;
     int seq_words;
+    int reg;  // register-wavefront chunks (0: offsets staged in LDS)
     size_t lds;
     uint32_t grid;
     bool persistent;
 };
 
-int seq_words_for(uint32_t max_len) { return (int)(max_len / 32u) + 2; }
+// 64-bit words of one sequence slot in LDS: the data plus two pad words.
+int seq_words_for(uint32_t max_len) { return (int)(max_len / 32u) + 3; }
+
+// 64-bit words of one overlapped sequence copy (register-wavefront passes).
+int overlapped_words_for(uint32_t max_len) { return 2 * (int)(max_len / 32u + 1) + 2; }
 
 // Dynamic LDS a pass may use: the device maximum less room for the kernel's
 // static LDS.
@@ -141,12 +146,15 @@ int max_dcap_for(int seq_words, int bt_lds, size_t budget) {
 }
 
 int finish_pass(ewfa_ctx* c, Pass& p, uint32_t items) {
-    p.lds = ewfa::lds_bytes(p.seq_words, p.dcap, p.bt_lds);
+    p.lds = p.reg ? ewfa::lds_bytes_reg(p.seq_words, p.dcap) : ewfa::lds_bytes(p.seq_words, p.dcap, p.bt_lds);
     if (p.lds > lds_budget(c))
         return fail(EWFA_ERR_RANGE, "pass needs %zu bytes of LDS (device has %zu)", p.lds, c->lds_max);
-    if (p.lds > 65536) HIPCHK(ewfa::align_set_max_lds(p.waves, p.lds));
+    if (p.lds > 65536) {
+        if (p.reg) return fail(EWFA_ERR_RANGE, "register pass above 64 KiB of LDS");
+        HIPCHK(ewfa::align_set_max_lds(p.waves, p.lds));
+    }
     if (p.persistent) {
-        int bpc = ewfa::align_blocks_per_cu(p.waves, p.lds);
+        int bpc = p.reg ? ewfa::align_reg_blocks_per_cu(p.reg, p.lds) : ewfa::align_blocks_per_cu(p.waves, p.lds);
         if (bpc < 1) bpc = 1;
         uint64_t g = (uint64_t)bpc * (uint64_t)c->num_cus;
         if (items > 0 && g > items) g = items;
@@ -181,6 +189,15 @@ int main_pass(ewfa_ctx* c, uint32_t max_len, uint32_t items, bool persistent, Pa
             return fail(EWFA_ERR_RANGE, "sequences of %u bases do not fit in LDS", max_len);
         if (p.dcap > most) p.dcap = most;
         if (p.dcap < 1) p.dcap = 1;
+        // one wave with the backtrace in LDS: offsets in registers when they fit
+        if (p.waves == 1 && p.bt_lds) {
+            const int nc = ewfa::reg_chunks_for(p.dcap);
+            const int ow = overlapped_words_for(max_len);
+            if (nc > 0 && ewfa::lds_bytes_reg(ow, p.dcap) <= 65536) {
+                p.reg = nc;
+                p.seq_words = ow;
+            }
+        }
     }
     return finish_pass(c, p, items);
 }
@@ -242,6 +259,18 @@ void fill_pass_args(ewfa::AlignArgs& a, const Pass& p, ewfa_ctx* c) {
     a.bt_slot_words = 2u * (uint64_t)ewfa::bt_chunks((uint32_t)p.dcap);
 }
 
+// The emit kernel for a pass: sequences staged in LDS when they fit (the
+// persistent path), read from global memory otherwise (and on the plain
+// path, which keeps the reference's one-lane-per-pair global walk).
+int emit(ewfa_ctx* c, const ewfa::EmitArgs& e, uint32_t items, uint32_t max_len, bool staged, hipStream_t s) {
+    const int sw = seq_words_for(max_len);
+    if (staged && sw <= ewfa::EMIT_LDS_MAX_SW)
+        HIPCHK(ewfa::launch_emit_lds(e, items, sw, s));
+    else
+        HIPCHK(ewfa::launch_emit(e, items, s));
+    return EWFA_OK;
+}
+
 // Enqueues the main pass plus emit for items [first, first+count) of the
 // identity list, deferrals collected in list_a / ctrl[CTRL_DEFER_A].
 int enqueue_main(ewfa_ctx* c, const Pass& p, const uint64_t* words, const ewfa_pair_t* pairs,
@@ -262,7 +291,10 @@ int enqueue_main(ewfa_ctx* c, const Pass& p, const uint64_t* words, const ewfa_p
     a.defer_count = ctrl + CTRL_DEFER_A;
     HIPCHK(ewfa::launch_fill_u32(ctrl, 0u, 3, s));
     if (timing) HIPCHK(hipEventRecord(c->ev_a0, s));
-    HIPCHK(ewfa::launch_align(a, p.waves, p.persistent, p.grid, p.lds, s));
+    if (p.reg)
+        HIPCHK(ewfa::launch_align_reg(a, p.reg, p.persistent, p.grid, p.lds, s));
+    else
+        HIPCHK(ewfa::launch_align(a, p.waves, p.persistent, p.grid, p.lds, s));
     if (timing) HIPCHK(hipEventRecord(c->ev_a1, s));
     ewfa::EmitArgs e{};
     e.words = words;
@@ -274,7 +306,7 @@ int enqueue_main(ewfa_ctx* c, const Pass& p, const uint64_t* words, const ewfa_p
     e.list = nullptr;
     e.count = count;
     e.count_ptr = nullptr;
-    HIPCHK(ewfa::launch_emit(e, count, s));
+    CHK(emit(c, e, count, max_len, p.persistent, s));
     if (timing) HIPCHK(hipEventRecord(c->ev_e1, s));
     HIPCHK(hipMemcpyAsync(c->host_ctrl + CTRL_DEFER_A, ctrl + CTRL_DEFER_A, 4,
                           hipMemcpyDeviceToHost, s));
@@ -332,7 +364,7 @@ int resolve_deferred(ewfa_ctx* c, int prev_dcap, const uint64_t* words, const ew
         e.cigars = cigars;
         e.list = in_list;
         e.count = pending;
-        HIPCHK(ewfa::launch_emit(e, pending, s));
+        CHK(emit(c, e, pending, max_len, true, s));
         HIPCHK(hipMemcpyAsync(c->host_ctrl + out_ctr, ctrl + out_ctr, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         pending = c->host_ctrl[out_ctr];

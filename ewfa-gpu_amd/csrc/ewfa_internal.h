@@ -89,10 +89,28 @@ inline size_t lds_bytes(int seq_words, int dcap, int bt_lds) {
     return b;
 }
 
+// LDS bytes of a register-wavefront slot (sequences + backtrace only).
+inline size_t lds_bytes_reg(int seq_words, int dcap) {
+    return 16u * (size_t)seq_words + 16u * (size_t)bt_chunks((uint32_t)dcap);
+}
+
+// Register chunks the register wavefront needs for dcap levels (0: too many).
+inline int reg_chunks_for(int dcap) {
+    const int nc = (2 * dcap + 1 + 63) / 64;
+    return nc <= 4 ? (nc < 1 ? 1 : nc) : 0;
+}
+
 // Kernel launchers (kernels.hip).  All asynchronous on `stream`.
 hipError_t launch_align(const AlignArgs& a, int waves, bool persistent, uint32_t grid,
                         size_t lds, hipStream_t stream);
+hipError_t launch_align_reg(const AlignArgs& a, int nc, bool persistent, uint32_t grid, size_t lds,
+                            hipStream_t stream);
+int align_reg_blocks_per_cu(int nc, size_t lds);
 hipError_t launch_emit(const EmitArgs& a, uint32_t grid_items, hipStream_t stream);
+// Emit with the sequences staged in LDS: sw words per sequence slot, 64 pairs
+// per block (64 * 2 * sw * 8 bytes of LDS).
+hipError_t launch_emit_lds(const EmitArgs& a, uint32_t grid_items, int sw, hipStream_t stream);
+constexpr int EMIT_LDS_MAX_SW = 64;  // 64 KiB of LDS per block
 hipError_t launch_pack(const char* ascii, size_t ascii_stride, const uint32_t* len, size_t nseq,
                        uint64_t* words, size_t words_stride, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);

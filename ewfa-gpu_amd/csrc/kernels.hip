@@ -57,6 +57,137 @@ __device__ __forceinline__ int extend(Ptr pat, Ptr txt, int plen, int tlen, int 
     return acc;
 }
 
+// 32 bases from base i of an LDS sequence copy (words i>>5 and i>>5+1 must be
+// readable; the copy carries a zero pad word).  Branchless funnel shift: the
+// second word moves right by 64-s as (b >> 1) >> (63 - s), which is 0 for s = 0.
+__device__ __forceinline__ uint64_t window32(const uint64_t* S, int i) {
+    const int w = i >> 5;
+    const uint32_t s = (uint32_t)(i & 31) << 1;
+    const uint64_t a = S[w];
+    const uint64_t b = S[w + 1];
+    return (a << s) | ((b >> 1) >> (63u - s));
+}
+
+// One 32-base comparison step from pattern position v, text position h:
+// the number of equal bases before the first mismatch or either end (0..32).
+// live == false (no base left, or an inactive lane) gives 0 without reading
+// past the copies.
+__device__ __forceinline__ int step32(const uint64_t* P, const uint64_t* T, int plen, int tlen, int v, int h,
+                                      bool live) {
+    const int rem = min(plen - v, tlen - h);
+    live = live && rem > 0 && v >= 0 && h >= 0;
+    const uint64_t x0 = window32(P, live ? v : 0) ^ window32(T, live ? h : 0);
+    const int r = live ? min(rem, 32) : 0;
+    const uint64_t x = x0 | ((r >= 32) ? 0ull : (~0ull >> (2 * r)));
+    return x ? (int)(__clzll(x) >> 1) : 32;
+}
+
+// Full extension from (v, h) on LDS copies.
+__device__ __forceinline__ int extend_lds(const uint64_t* P, const uint64_t* T, int plen, int tlen, int v, int h) {
+    int acc = 0;
+    for (;;) {
+        const int e = step32(P, T, plen, tlen, v + acc, h + acc, true);
+        acc += e;
+        if (e < 32) break;
+    }
+    return acc;
+}
+
+// ---- LDS copies as 32-bit words ------------------------------------------
+// The align and emit kernels keep their LDS copies as 32-bit words in base
+// order (word q = bases 16q..16q+15, first base in the high bits), written
+// by swapping the halves of each 64-bit global word.  Two zero pad words
+// (four 32-bit words) follow the data.
+
+// Global 64-bit word -> the two 32-bit LDS words it holds, as one 64-bit store.
+__device__ __forceinline__ uint64_t to_lds_order(uint64_t w) { return (w >> 32) | (w << 32); }
+
+// 32 bases from base i (i >= 0 and at most 32 words from the copy's end).
+__device__ __forceinline__ uint64_t win32(const uint32_t* S, int i) {
+    const int w = i >> 4;
+    const uint32_t s = (uint32_t)(i & 15) << 1;
+    const uint32_t a = S[w], b = S[w + 1], c = S[w + 2];
+    const uint64_t x1 = ((uint64_t)a << 32) | b;
+    const uint64_t x2 = ((uint64_t)b << 32) | c;
+    return ((x1 << s) & 0xFFFFFFFF00000000ull) | ((x2 << s) >> 32);
+}
+
+// Equal leading bases of two 32-base windows, 0..32.
+__device__ __forceinline__ int lead_eq(uint64_t x) {
+    const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
+    const uint32_t z = hi ? (uint32_t)__clz((int)hi) : 32u + (uint32_t)__clz((int)lo);  // __clz(0) = 32
+    return (int)(z >> 1);
+}
+
+// One 32-base step from pattern position v, text position h (clamped into
+// the copies): equal bases before the first mismatch, capped at the bases
+// left before either end (0..32).  Inactive lanes give 0.
+__device__ __forceinline__ int step32w(const uint32_t* P, const uint32_t* T, int plen, int tlen, int v, int h,
+                                       bool live) {
+    const int rem = live ? max(min(plen - v, tlen - h), 0) : 0;
+    const int vc = min(max(v, 0), plen), hc = min(max(h, 0), tlen);
+    return min(lead_eq(win32(P, vc) ^ win32(T, hc)), rem);
+}
+
+__device__ __forceinline__ int extend_w(const uint32_t* P, const uint32_t* T, int plen, int tlen, int v, int h) {
+    int acc = 0;
+    for (;;) {
+        const int e = step32w(P, T, plen, tlen, v + acc, h + acc, true);
+        acc += e;
+        if (e < 32) break;
+    }
+    return acc;
+}
+
+// ---- overlapped LDS copies (align kernel) ---------------------------------
+// Word q of an overlapped copy holds 32-bit words q and q+1 of the sequence
+// (bases 16q..16q+31, first base in the high bits).  From the global 64-bit
+// words G: word 2m = G[m], word 2m+1 = (G[m] << 32) | (G[m+1] >> 32).  A
+// copy of a len-base sequence takes 2*ceil(len/32) + 2 words (the last two
+// zero), so any window starting at or before len is readable.
+
+// 32 bases from base i: one ds_read2_b64, two 64-bit shifts; the high halves
+// of the shifted words are the window.
+__device__ __forceinline__ uint64_t win32o(const uint64_t* S, int i) {
+    const int w = i >> 4;
+    const uint32_t s = (uint32_t)(i & 15) << 1;
+    const uint64_t x1 = S[w] << s;
+    const uint64_t x2 = S[w + 1] << s;
+    return (x1 & 0xFFFFFFFF00000000ull) | (x2 >> 32);
+}
+
+// One 32-base step: matched bases from pattern position v and text position
+// h, capped at the bases left before either end (0 when inactive).
+__device__ __forceinline__ int step32o(const uint64_t* P, const uint64_t* T, int plen, int tlen, int v, int h,
+                                       bool live) {
+    const int rem = live ? max(min(plen - v, tlen - h), 0) : 0;
+    const int vc = min(max(v, 0), plen), hc = min(max(h, 0), tlen);
+    const int eq = lead_eq(win32o(P, vc) ^ win32o(T, hc));
+    return min(eq, rem);
+}
+
+__device__ __forceinline__ int extend_o(const uint64_t* P, const uint64_t* T, int plen, int tlen, int v, int h) {
+    int acc = 0;
+    for (;;) {
+        const int e = step32o(P, T, plen, tlen, v + acc, h + acc, true);
+        acc += e;
+        if (e < 32) break;
+    }
+    return acc;
+}
+
+// Stages one sequence of len bases from global words g into an overlapped
+// copy at S (2*ceil(len/32) + 2 words), cooperatively over 64 lanes.
+__device__ __forceinline__ void stage_overlapped(uint64_t* S, const uint64_t* g, int len, int lane) {
+    const int nw = (len + 31) >> 5;
+    for (int m = lane; m <= nw; m += 64) {
+        const uint64_t a = m < nw ? g[m] : 0ull;
+        const uint64_t b = m + 1 < nw ? g[m + 1] : 0ull;
+        S[2 * m] = a;
+        S[2 * m + 1] = (a << 32) | (b >> 32);
+    }
+}
+
 // Synchronisation of the threads working on one pair.
 template <int G>
 __device__ __forceinline__ void group_sync() {
@@ -77,103 +208,13 @@ __device__ __forceinline__ int slot_bt_word(int seq_words, int dcap) {
     return 2 * seq_words + ((2 * dcap + 5 + 1) & ~1);
 }
 
-template <int G, bool BT_LDS>
-__device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, uint64_t* bt_g,
-                           int gtid) {
-    const int lane = gtid & 63;
-    const int wave = gtid >> 6;
-    const ewfa_pair_t pr = a.pairs[pidx];
-    const int plen = (int)pr.plen, tlen = (int)pr.tlen;
+// End of a pair's search: status, deferral, traceback into the ops scratch.
+// Called by every thread of the group with the same (done, d, limit).
+__device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, const uint64_t* BT, int gtid,
+                                            bool done, int d, int limit, int tk) {
     ewfa_result_t* res = a.results + pidx;
-    const int sw = a.seq_words;
-    const int nwp = (plen + 31) >> 5, nwt = (tlen + 31) >> 5;
-    if (pr.plen > a.max_len || pr.tlen > a.max_len || nwp + 1 > sw || nwt + 1 > sw) {
-        if (gtid == 0) *res = ewfa_result_t{0, 0, EWFA_PAIR_TOO_LONG, 0};
-        return;
-    }
-    const int dcap = a.dcap;
-    uint64_t* P = slot;
-    uint64_t* T = slot + sw;
-    int32_t* X = reinterpret_cast<int32_t*>(slot + 2 * sw) + (dcap + 2);
-    int32_t* Y = X + (2 * dcap + 5);
-    uint64_t* BT = BT_LDS ? slot + slot_bt_word(sw, dcap) : bt_g;
-
-    // stage the two sequences (coalesced 8-byte loads) plus a zero pad word
-    const uint64_t* gp = a.words + pr.pattern_off;
-    const uint64_t* gt = a.words + pr.text_off;
-    for (int i = gtid; i < nwp; i += 64 * G) P[i] = gp[i];
-    for (int i = gtid; i < nwt; i += 64 * G) T[i] = gt[i];
-    if (gtid == 0) {
-        P[nwp] = 0;
-        T[nwt] = 0;
-    }
-    group_sync<G>();
-
-    const int tk = tlen - plen;  // target diagonal, target offset tlen
     const int tka = tk < 0 ? -tk : tk;
-    if (gtid == 0) {
-        X[0] = extend<false>(P, T, plen, tlen, 0, 0);
-        X[-1] = -1;
-        X[1] = -1;
-        X[-2] = -1;
-        X[2] = -1;
-    }
-    group_sync<G>();
-
     const int cap = a.max_distance;
-    int limit = dcap;
-    if (cap > 0 && cap - 1 < limit) limit = cap - 1;
-    bool done = (tk == 0) && (X[0] == tlen);
-    int d = 0;
-    int32_t* cur = X;
-    int32_t* nxt = Y;
-    while (!done && d < limit) {
-        ++d;
-        const int nch = (d >> 5) + 1;
-        const uint32_t base = bt_level_base((uint32_t)d);
-        for (int c = wave; c < nch; c += G) {
-            const int idx = (c << 6) + lane;  // diagonal index 0..2d
-            const int k = idx - d;
-            int op = 0;
-            if (idx <= 2 * d) {
-                // candidates from level d-1 (src/kernels.cu:131-138); the max
-                // of (value, op) with DEL(3) > SUB(2) > INS(1) on ties
-                // (src/kernels.cu:141-152)
-                const int del = cur[k + 1];
-                const int sub = cur[k] + 1;
-                const int ins = cur[k - 1] + 1;
-                int val = del;
-                op = OP_DEL;
-                if (sub > val) {
-                    val = sub;
-                    op = OP_SUB;
-                }
-                if (ins > val) {
-                    val = ins;
-                    op = OP_INS;
-                }
-                nxt[k] = val + extend<false>(P, T, plen, tlen, val - k, val);
-            }
-            const uint64_t lo = __ballot(op & 1);
-            const uint64_t hi = __ballot(op & 2);
-            if (lane == 0) {
-                uint64_t* q = BT + 2 * (size_t)(base + (uint32_t)c);
-                q[0] = lo;
-                q[1] = hi;
-            }
-        }
-        // sentinels for the next level's reads at |k| = d+1, d+2
-        if (gtid < 4) {
-            const int e = gtid < 2 ? -(d + 1 + gtid) : (d + 1 + (gtid - 2));
-            nxt[e] = -1;
-        }
-        group_sync<G>();
-        done = (tka <= d) && (nxt[tk] == tlen);
-        int32_t* tmp = cur;
-        cur = nxt;
-        nxt = tmp;
-    }
-
     int status = EWFA_PAIR_OK;
     int dist = d, start = d;
     if (!done) {
@@ -224,6 +265,398 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
     }
 }
 
+// finish_pair for one wave whose backtrace rows hold at most NC chunks per
+// level: the 64 lanes load the rows of 64 consecutive levels at once (lane j
+// holds level top-j), then the walk takes each op bit out of lane j with
+// readlane, so the serial part runs on scalar registers.
+template <int NC>
+__device__ __forceinline__ void finish_pair_rows(const AlignArgs& a, uint32_t pidx, const uint64_t* BT, int lane,
+                                                 bool done, int d, int limit, int tk) {
+    ewfa_result_t* res = a.results + pidx;
+    const int tka = tk < 0 ? -tk : tk;
+    const int cap = a.max_distance;
+    int status = EWFA_PAIR_OK;
+    int dist = d, start = d;
+    if (!done) {
+        if (cap > 0 && limit == cap - 1) {
+            // the reference's cap: distance = cap, ops of level cap-1 at the
+            // target diagonal (src/kernels.cu:259, :288-303)
+            status = tka > cap ? EWFA_PAIR_UNDEFINED : EWFA_PAIR_CAPPED;
+            dist = cap;
+            start = cap - 1;
+        } else {
+            if (lane == 0) {
+                const uint32_t at = atomicAdd(a.defer_count, 1u);
+                a.defer_list[at] = pidx;
+                res->status = PAIR_DEFERRED;
+            }
+            return;
+        }
+    }
+    uint64_t* ow = a.ops + (size_t)pidx * a.ops_stride;
+    if (dist > 0) {
+        int wi = (dist - 1) >> 5;
+        uint64_t word = 0;
+        int k = tk;
+        const int last = tka <= start ? 1 : start + 1;  // no walk when |t| > start
+        for (int top = start; top >= last; top -= 64) {
+            const int lev_l = top - lane;
+            uint64_t rlo[NC], rhi[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                rlo[c] = 0;
+                rhi[c] = 0;
+            }
+            if (lev_l >= 1) {
+                const uint64_t* row = BT + 2 * (size_t)bt_level_base((uint32_t)lev_l);
+                const int nch = (lev_l >> 5) + 1;
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    if (c < nch) {
+                        rlo[c] = row[2 * c];
+                        rhi[c] = row[2 * c + 1];
+                    }
+            }
+            const int steps = min(64, top - last + 1);
+            for (int j = 0; j < steps; ++j) {
+                const int lev = top - j;
+                const int s = lev - 1;
+                while (wi > (s >> 5)) {
+                    if (lane == 0) ow[wi] = word;
+                    --wi;
+                    word = 0;
+                }
+                const int idx = k + lev;
+                const int c = idx >> 6, bit = idx & 63;
+                uint32_t wl = 0, wh = 0;
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc)
+                    if (cc == c) {
+                        const uint64_t l64 = rlo[cc], h64 = rhi[cc];
+                        if (bit >= 32) {
+                            wl = __builtin_amdgcn_readlane((int)(uint32_t)(l64 >> 32), j);
+                            wh = __builtin_amdgcn_readlane((int)(uint32_t)(h64 >> 32), j);
+                        } else {
+                            wl = __builtin_amdgcn_readlane((int)(uint32_t)l64, j);
+                            wh = __builtin_amdgcn_readlane((int)(uint32_t)h64, j);
+                        }
+                    }
+                const int b = bit & 31;
+                const int op = (int)((wl >> b) & 1u) | (int)(((wh >> b) & 1u) << 1);
+                word |= (uint64_t)op << ((s & 31) << 1);
+                k += op - 2;
+            }
+        }
+        while (wi >= 0) {
+            if (lane == 0) ow[wi] = word;
+            --wi;
+            word = 0;
+        }
+    }
+    if (lane == 0) *res = ewfa_result_t{dist, 0, status, 0};
+}
+
+template <int G, bool BT_LDS>
+__device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, uint64_t* bt_g,
+                           int gtid) {
+    const int lane = gtid & 63;
+    const int wave = gtid >> 6;
+    const ewfa_pair_t pr = a.pairs[pidx];
+    const int plen = (int)pr.plen, tlen = (int)pr.tlen;
+    ewfa_result_t* res = a.results + pidx;
+    const int sw = a.seq_words;
+    const int nwp = (plen + 31) >> 5, nwt = (tlen + 31) >> 5;
+    if (pr.plen > a.max_len || pr.tlen > a.max_len || nwp + 1 > sw || nwt + 1 > sw) {
+        if (gtid == 0) *res = ewfa_result_t{0, 0, EWFA_PAIR_TOO_LONG, 0};
+        return;
+    }
+    const int dcap = a.dcap;
+    uint64_t* P = slot;
+    uint64_t* T = slot + sw;
+    int32_t* X = reinterpret_cast<int32_t*>(slot + 2 * sw) + (dcap + 2);
+    int32_t* Y = X + (2 * dcap + 5);
+    uint64_t* BT = BT_LDS ? slot + slot_bt_word(sw, dcap) : bt_g;
+
+    // stage the two sequences (coalesced 8-byte loads) plus a zero pad word
+    const uint64_t* gp = a.words + pr.pattern_off;
+    const uint64_t* gt = a.words + pr.text_off;
+    for (int i = gtid; i < nwp; i += 64 * G) P[i] = gp[i];
+    for (int i = gtid; i < nwt; i += 64 * G) T[i] = gt[i];
+    if (gtid == 0) {
+        P[nwp] = 0;
+        T[nwt] = 0;
+    }
+    group_sync<G>();
+
+    const int tk = tlen - plen;  // target diagonal, target offset tlen
+    const int tka = tk < 0 ? -tk : tk;
+    if (gtid == 0) {
+        X[0] = extend_lds(P, T, plen, tlen, 0, 0);
+        X[-1] = -1;
+        X[1] = -1;
+        X[-2] = -1;
+        X[2] = -1;
+    }
+    group_sync<G>();
+
+    const int cap = a.max_distance;
+    int limit = dcap;
+    if (cap > 0 && cap - 1 < limit) limit = cap - 1;
+    bool done = (tk == 0) && (X[0] == tlen);
+    int d = 0;
+    int32_t* cur = X;
+    int32_t* nxt = Y;
+    while (!done && d < limit) {
+        ++d;
+        const int nch = (d >> 5) + 1;
+        const uint32_t base = bt_level_base((uint32_t)d);
+        for (int c = wave; c < nch; c += G) {
+            const int idx = (c << 6) + lane;  // diagonal index 0..2d
+            const int k = idx - d;
+            int op = 0;
+            if (idx <= 2 * d) {
+                // candidates from level d-1 (src/kernels.cu:131-138); the max
+                // of (value, op) with DEL(3) > SUB(2) > INS(1) on ties
+                // (src/kernels.cu:141-152)
+                const int del = cur[k + 1];
+                const int sub = cur[k] + 1;
+                const int ins = cur[k - 1] + 1;
+                int val = del;
+                op = OP_DEL;
+                if (sub > val) {
+                    val = sub;
+                    op = OP_SUB;
+                }
+                if (ins > val) {
+                    val = ins;
+                    op = OP_INS;
+                }
+                nxt[k] = val + extend_lds(P, T, plen, tlen, val - k, val);
+            }
+            const uint64_t lo = __ballot(op & 1);
+            const uint64_t hi = __ballot(op & 2);
+            if (lane == 0) {
+                uint64_t* q = BT + 2 * (size_t)(base + (uint32_t)c);
+                q[0] = lo;
+                q[1] = hi;
+            }
+        }
+        // sentinels for the next level's reads at |k| = d+1, d+2
+        if (gtid < 4) {
+            const int e = gtid < 2 ? -(d + 1 + gtid) : (d + 1 + (gtid - 2));
+            nxt[e] = -1;
+        }
+        group_sync<G>();
+        done = (tka <= d) && (nxt[tk] == tlen);
+        int32_t* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+
+    finish_pair(a, pidx, BT, gtid, done, d, limit, tk);
+}
+
+// One level of the register wavefront with NA active chunks: candidates for
+// every chunk, the first 32-base compare of every chunk back to back, then a
+// continuation loop only while some lane matched all 32 bases.
+template <int NA, int NC>
+__device__ __forceinline__ void reg_level(int (&o)[NC], int d, int lane, const uint64_t* P, const uint64_t* T,
+                                          int plen, int tlen, uint64_t* BTlev) {
+    int p62[NA], p63[NA];
+#pragma unroll
+    for (int c = 0; c + 1 < NA; ++c) {
+        p62[c] = __builtin_amdgcn_readlane(o[c], 62);
+        p63[c] = __builtin_amdgcn_readlane(o[c], 63);
+    }
+    int val[NA], op[NA], acc[NA];
+    bool valid[NA];
+#pragma unroll
+    for (int c = 0; c < NA; ++c) {
+        const int in63 = c ? p63[c - 1] : -1;
+        const int in62 = c ? p62[c - 1] : -1;
+        const int s1 = __builtin_amdgcn_update_dpp(in63, o[c], 0x138, 0xF, 0xF, false);
+        const int s2 = __builtin_amdgcn_update_dpp(in62, s1, 0x138, 0xF, 0xF, false);
+        const int idx = (c << 6) + lane;
+        // candidates (src/kernels.cu:131-138) packed as 4*value + op, so one
+        // max gives the value and DEL(3) > SUB(2) > INS(1) on ties
+        // (src/kernels.cu:141-152)
+        const int m = max(max(o[c] * 4 + OP_DEL, s1 * 4 + (4 + OP_SUB)), s2 * 4 + (4 + OP_INS));
+        valid[c] = idx <= 2 * d;
+        val[c] = m >> 2;
+        op[c] = valid[c] ? (m & 3) : 0;
+        acc[c] = step32o(P, T, plen, tlen, val[c] - (idx - d), val[c], valid[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NA; ++c) {
+        int last = acc[c];
+        const int k = (c << 6) + lane - d;
+        while (__ballot(last == 32)) {
+            const bool go = last == 32;
+            const int e = step32o(P, T, plen, tlen, val[c] - k + acc[c], val[c] + acc[c], go);
+            acc[c] += go ? e : 0;
+            last = go ? e : 0;
+        }
+    }
+    uint64_t lo_sel = 0, hi_sel = 0;
+#pragma unroll
+    for (int c = 0; c < NA; ++c) {
+        o[c] = valid[c] ? val[c] + acc[c] : -1;
+        const uint64_t lo = __ballot(op[c] & 1);
+        const uint64_t hi = __ballot(op[c] & 2);
+        if (lane == c) {
+            lo_sel = lo;
+            hi_sel = hi;
+        }
+    }
+    // lane c stores chunk c's two ballots
+    if (lane < NA) {
+        uint64_t* q = BTlev + 2 * lane;
+        q[0] = lo_sel;
+        q[1] = hi_sel;
+    }
+}
+
+// A pair's descriptor and, when both sequences total at most 64 words, its
+// words in registers (lane j: pattern word j, then text words), loaded ahead
+// of use so the global latency overlaps the previous pair's work.
+struct PairPrefetch {
+    ewfa_pair_t pr;
+    uint64_t word;
+    int in_regs;
+};
+
+__device__ __forceinline__ PairPrefetch prefetch_pair(const AlignArgs& a, uint32_t pidx, int lane) {
+    PairPrefetch f;
+    f.pr = a.pairs[pidx];
+    const int nwp = ((int)f.pr.plen + 31) >> 5, nwt = ((int)f.pr.tlen + 31) >> 5;
+    f.in_regs = (nwp + nwt <= 64) ? 1 : 0;
+    f.word = 0;
+    if (f.in_regs) {
+        if (lane < nwp) f.word = a.words[f.pr.pattern_off + lane];
+        else if (lane < nwp + nwt) f.word = a.words[f.pr.text_off + (lane - nwp)];
+    }
+    return f;
+}
+
+// Overlapped copies of both sequences from prefetched registers: lane j's
+// next word arrives from lane j+1 by a DPP whole-wave left shift.
+__device__ __forceinline__ void stage_from_regs(uint64_t* P, uint64_t* T, const PairPrefetch& f, int lane) {
+    const int nwp = ((int)f.pr.plen + 31) >> 5, nwt = ((int)f.pr.tlen + 31) >> 5;
+    const uint32_t lo = (uint32_t)f.word, hi = (uint32_t)(f.word >> 32);
+    const uint32_t nlo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x130, 0xF, 0xF, false);
+    const uint32_t nhi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x130, 0xF, 0xF, false);
+    const bool is_p = lane < nwp;
+    const int m = is_p ? lane : lane - nwp;
+    const int nw = is_p ? nwp : nwt;
+    const uint64_t b = (m + 1 < nw) ? (((uint64_t)nhi << 32) | nlo) : 0ull;
+    uint64_t* S = is_p ? P : T;
+    if (lane < nwp + nwt) {
+        S[2 * m] = f.word;
+        S[2 * m + 1] = (f.word << 32) | (b >> 32);
+    }
+    if (lane < 2) {
+        P[2 * nwp + lane] = 0;
+        T[2 * nwt + lane] = 0;
+    }
+}
+
+// Register-resident wavefront for one wave per pair.  The offsets of a level
+// live in NC VGPRs: chunk c, lane j holds diagonal k = 64c + j - d.  The next
+// level reads the same lane (k+1), the lane before (k) and the lane two
+// before (k-1): two DPP whole-wave right shifts, with the chunk below's
+// lanes 62/63 entering lane 0/1 through readlane.  Diagonals outside [-d, d]
+// hold -1.  Levels up to 32*NC-1 fit.
+template <int NC>
+__device__ __forceinline__ void align_pair_reg(const AlignArgs& a, uint32_t pidx, const PairPrefetch& f, uint64_t* slot, int lane) {
+    const ewfa_pair_t& pr = f.pr;
+    const int plen = (int)pr.plen, tlen = (int)pr.tlen;
+    const int sw = a.seq_words;
+    const int nwp = (plen + 31) >> 5, nwt = (tlen + 31) >> 5;
+    // sw: words of one overlapped sequence copy
+    if (pr.plen > a.max_len || pr.tlen > a.max_len || 2 * nwp + 2 > sw || 2 * nwt + 2 > sw) {
+        if (lane == 0) a.results[pidx] = ewfa_result_t{0, 0, EWFA_PAIR_TOO_LONG, 0};
+        return;
+    }
+    uint64_t* P = slot;
+    uint64_t* T = slot + sw;
+    uint64_t* BT = slot + 2 * sw;
+    if (f.in_regs) {
+        stage_from_regs(P, T, f, lane);
+    } else {
+        stage_overlapped(P, a.words + pr.pattern_off, plen, lane);
+        stage_overlapped(T, a.words + pr.text_off, tlen, lane);
+    }
+    group_sync<1>();
+
+    const int tk = tlen - plen;
+    const int tka = tk < 0 ? -tk : tk;
+    int o[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) o[c] = -1;
+    const int o00 = extend_o(P, T, plen, tlen, 0, 0);
+    if (lane == 0) o[0] = o00;
+
+    int limit = min(a.dcap, 32 * NC - 1);
+    const int cap = a.max_distance;
+    if (cap > 0 && cap - 1 < limit) limit = cap - 1;
+    bool done = (tk == 0) && (o00 == tlen);
+    int d = 0;
+    while (!done && d < limit) {
+        ++d;
+        uint64_t* BTlev = BT + 2 * (size_t)bt_level_base((uint32_t)d);
+        const int nch = (d >> 5) + 1;
+        if (nch == 1) reg_level<1, NC>(o, d, lane, P, T, plen, tlen, BTlev);
+        else if (NC >= 2 && nch == 2) reg_level<(NC >= 2 ? 2 : 1), NC>(o, d, lane, P, T, plen, tlen, BTlev);
+        else if (NC >= 3 && nch == 3) reg_level<(NC >= 3 ? 3 : 1), NC>(o, d, lane, P, T, plen, tlen, BTlev);
+        else if (NC >= 4) reg_level<(NC >= 4 ? 4 : 1), NC>(o, d, lane, P, T, plen, tlen, BTlev);
+        if (tka <= d) {
+            const int ti = tk + d;  // flattened index of the target diagonal
+            int tv = -1;
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if ((ti >> 6) == c) tv = __builtin_amdgcn_readlane(o[c], ti & 63);
+            done = (tv == tlen);
+        }
+    }
+    group_sync<1>();
+    finish_pair_rows<NC>(a, pidx, BT, lane, done, d, limit, tk);
+}
+
+template <int NC>
+__global__ __launch_bounds__(64) void align_reg_kernel(AlignArgs a, int persistent) {
+    extern __shared__ uint64_t lds[];
+    const int lane = threadIdx.x;
+    const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
+    if (!persistent) {
+        const uint32_t item = blockIdx.x;
+        if (item < count) {
+            const uint32_t pidx = a.list ? a.list[item] : item;
+            const PairPrefetch f = prefetch_pair(a, pidx, lane);
+            align_pair_reg<NC>(a, pidx, f, lds, lane);
+        }
+        return;
+    }
+    for (;;) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(a.queue, a.grab);
+        const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+        if (base >= count) break;
+        const uint32_t end = min(base + a.grab, count);
+        uint32_t pidx = a.list ? a.list[base] : base;
+        PairPrefetch cur = prefetch_pair(a, pidx, lane);
+        for (uint32_t it = base; it < end; ++it) {
+            const bool more = it + 1 < end;
+            const uint32_t nidx = more ? (a.list ? a.list[it + 1] : it + 1) : 0u;
+            PairPrefetch nxt = cur;
+            if (more) nxt = prefetch_pair(a, nidx, lane);  // in flight during this pair
+            align_pair_reg<NC>(a, pidx, cur, lds, lane);
+            group_sync<1>();
+            cur = nxt;
+            pidx = nidx;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // align kernel: G waves per pair, one pair slot per block
 
@@ -244,7 +677,7 @@ __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persiste
         if constexpr (G == 1) {
             uint32_t b = 0;
             if (tid == 0) b = atomicAdd(a.queue, a.grab);
-            base = __shfl(b, 0);
+            base = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
         } else {
             if (tid == 0) s_base = atomicAdd(a.queue, a.grab);
             __syncthreads();
@@ -345,6 +778,125 @@ __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a) {
     if (w.len > pr.cigar_cap) a.results[i].status = EWFA_PAIR_OVERFLOW;
 }
 
+// Same walk with the sequences of the block's 64 pairs staged in LDS first:
+// the wave loads each pair's words with coalesced reads, then every lane
+// walks its own pair out of LDS.  sw words per sequence slot.
+__global__ __launch_bounds__(64) void emit_lds_kernel(EmitArgs a, int sw) {
+    extern __shared__ uint64_t lds[];
+    const int lane = threadIdx.x;
+    const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
+    const uint32_t item0 = blockIdx.x * 64u;
+    if (item0 >= count) return;
+    const uint32_t item = item0 + (uint32_t)lane;
+    const bool mine = item < count;
+    const uint32_t i = mine ? (a.list ? a.list[item] : item) : 0u;
+    ewfa_pair_t pr{};
+    ewfa_result_t r{0, 0, -1, 0};
+    if (mine) {
+        pr = a.pairs[i];
+        r = a.results[i];
+    }
+    const bool want = mine && (r.status == EWFA_PAIR_OK || r.status == EWFA_PAIR_CAPPED ||
+                               r.status == EWFA_PAIR_UNDEFINED);
+    // stage: lane j loads word j of pair p (pattern words then text words),
+    // 16 pairs per round so that 16 loads per lane are in flight
+    const uint32_t nvalid = min(64u, count - item0);
+    const int nw_lane = ((int)pr.plen + 31) / 32 + ((int)pr.tlen + 31) / 32;
+    for (uint32_t p0 = 0; p0 < nvalid; p0 += 16) {
+        uint64_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const uint32_t p = p0 + u;
+            v[u] = 0;
+            if (p < nvalid) {
+                const int pl = __builtin_amdgcn_readlane((int)pr.plen, (int)p);
+                const int nwp = (pl + 31) >> 5;
+                const int nw = __builtin_amdgcn_readlane(nw_lane, (int)p);
+                const uint64_t po = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(pr.pattern_off >> 32), (int)p) << 32) |
+                                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pr.pattern_off, (int)p);
+                const uint64_t to = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(pr.text_off >> 32), (int)p) << 32) |
+                                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pr.text_off, (int)p);
+                if (lane < nw) v[u] = to_lds_order(a.words[lane < nwp ? po + lane : to + (lane - nwp)]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const uint32_t p = p0 + u;
+            if (p < nvalid) {
+                const int pl = __builtin_amdgcn_readlane((int)pr.plen, (int)p);
+                const int tl = __builtin_amdgcn_readlane((int)pr.tlen, (int)p);
+                const int nwp = (pl + 31) >> 5, nwt = (tl + 31) >> 5;
+                uint64_t* P = lds + (size_t)p * 2 * sw;
+                uint64_t* T = P + sw;
+                if (nwp + 2 <= sw && nwt + 2 <= sw) {
+                    if (lane < nwp) P[lane] = v[u];
+                    else if (lane < nwp + nwt) T[lane - nwp] = v[u];
+                    if (lane < 2) {
+                        P[nwp + lane] = 0;
+                        T[nwt + lane] = 0;
+                    }
+                    // words past the first 64 of a pair (sequences over 1 kbp)
+                    if (nwp + nwt > 64) {
+                        const uint64_t po = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(pr.pattern_off >> 32), (int)p) << 32) |
+                                            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pr.pattern_off, (int)p);
+                        const uint64_t to = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(pr.text_off >> 32), (int)p) << 32) |
+                                            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pr.text_off, (int)p);
+                        for (int w = lane + 64; w < nwp + nwt; w += 64) {
+                            if (w < nwp) P[w] = to_lds_order(a.words[po + w]);
+                            else T[w - nwp] = to_lds_order(a.words[to + (w - nwp)]);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    group_sync<1>();
+    if (!want) return;
+    if ((pr.cigar_off & 3u) || (pr.cigar_cap & 3u)) {
+        a.results[i].status = EWFA_PAIR_BADARG;
+        return;
+    }
+    const int plen = (int)pr.plen, tlen = (int)pr.tlen;
+    if (((plen + 31) >> 5) + 2 > sw || ((tlen + 31) >> 5) + 2 > sw) {
+        a.results[i].status = EWFA_PAIR_TOO_LONG;
+        return;
+    }
+    StreamWriter w{a.cigars + pr.cigar_off, pr.cigar_cap, 0u, 0u};
+    if (r.status != EWFA_PAIR_UNDEFINED) {
+        const uint32_t* P = reinterpret_cast<const uint32_t*>(lds + (size_t)lane * 2 * sw);
+        const uint32_t* T = P + 2 * sw;
+        const uint64_t* ow = a.ops + (size_t)i * a.ops_stride;
+        int k = 0, off = 0;
+        uint64_t word = 0;
+        bool stopped = false;
+        for (int s = 0; s < r.distance; ++s) {
+            const int e = extend_w(P, T, plen, tlen, off - k, off);
+            w.run(e);
+            off += e;
+            if ((s & 31) == 0) word = ow[s >> 5];
+            const int op = (int)((word >> ((s & 31) << 1)) & 3ull);
+            if (op == OP_DEL) {
+                w.put('D');
+                --k;
+            } else if (op == OP_SUB) {
+                w.put('X');
+                ++off;
+            } else if (op == OP_INS) {
+                w.put('I');
+                ++k;
+                ++off;
+            } else {
+                stopped = true;  // op 0 ends the walk (src/kernels.cu:471-478)
+                break;
+            }
+        }
+        if (!stopped) w.run(extend_w(P, T, plen, tlen, off - k, off));
+    }
+    w.finish();
+    a.results[i].cigar_len = (int32_t)w.len;
+    if (w.len > pr.cigar_cap) a.results[i].status = EWFA_PAIR_OVERFLOW;
+}
+
 // ---------------------------------------------------------------------------
 // pack and fill
 
@@ -402,6 +954,34 @@ hipError_t launch_align(const AlignArgs& a, int waves, bool persistent, uint32_t
     }
 }
 
+hipError_t launch_align_reg(const AlignArgs& a, int nc, bool persistent, uint32_t grid, size_t lds,
+                            hipStream_t stream) {
+    if (grid == 0) return hipSuccess;
+    const int p = persistent ? 1 : 0;
+    switch (nc) {
+        case 1: hipLaunchKernelGGL(align_reg_kernel<1>, dim3(grid), dim3(64), lds, stream, a, p); break;
+        case 2: hipLaunchKernelGGL(align_reg_kernel<2>, dim3(grid), dim3(64), lds, stream, a, p); break;
+        case 3: hipLaunchKernelGGL(align_reg_kernel<3>, dim3(grid), dim3(64), lds, stream, a, p); break;
+        case 4: hipLaunchKernelGGL(align_reg_kernel<4>, dim3(grid), dim3(64), lds, stream, a, p); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int align_reg_blocks_per_cu(int nc, size_t lds) {
+    const void* f = nullptr;
+    switch (nc) {
+        case 1: f = reinterpret_cast<const void*>(&align_reg_kernel<1>); break;
+        case 2: f = reinterpret_cast<const void*>(&align_reg_kernel<2>); break;
+        case 3: f = reinterpret_cast<const void*>(&align_reg_kernel<3>); break;
+        case 4: f = reinterpret_cast<const void*>(&align_reg_kernel<4>); break;
+        default: return 0;
+    }
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, 64, lds) != hipSuccess) return 0;
+    return blocks;
+}
+
 template <int G, bool BT_LDS>
 static const void* align_fn() {
     return reinterpret_cast<const void*>(&align_kernel<G, BT_LDS>);
@@ -440,6 +1020,13 @@ int align_blocks_per_cu(int waves, size_t lds) {
 hipError_t launch_emit(const EmitArgs& a, uint32_t grid_items, hipStream_t stream) {
     if (grid_items == 0) return hipSuccess;
     hipLaunchKernelGGL(emit_kernel, dim3((grid_items + 255) / 256), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit_lds(const EmitArgs& a, uint32_t grid_items, int sw, hipStream_t stream) {
+    if (grid_items == 0) return hipSuccess;
+    const size_t lds = (size_t)64 * 2 * (size_t)sw * 8;
+    hipLaunchKernelGGL(emit_lds_kernel, dim3((grid_items + 63) / 64), dim3(64), lds, stream, a, sw);
     return hipGetLastError();
 }
 
