@@ -70,7 +70,14 @@ struct DevBuf {
 };
 
 // Control words in device memory.
-enum : int { CTRL_QUEUE = 0, CTRL_DEFER_A = 1, CTRL_DEFER_B = 2, CTRL_WORDS = 64 };
+enum : int {
+    CTRL_QUEUE = 0,     // queue head of the plain/deferral passes
+    CTRL_DEFER_A = 1,   // deferred-pair counts (ping-pong)
+    CTRL_DEFER_B = 2,
+    CTRL_BQUEUE = 8,    // queue heads of the bucket passes (MAX_BUCKETS)
+    CTRL_BCOUNT = 16,   // pairs per bucket (MAX_BUCKETS)
+    CTRL_WORDS = 64
+};
 
 }  // namespace
 
@@ -83,7 +90,7 @@ struct ewfa_ctx {
     hipEvent_t ev_in = nullptr, ev_a0 = nullptr, ev_a1 = nullptr, ev_e1 = nullptr;
     int num_cus = 0;
     size_t lds_max = 0;
-    DevBuf ops, bt, ctrl, list_a, list_b;
+    DevBuf ops, bt, ctrl, list_a, list_b, buckets;
     uint32_t* host_ctrl = nullptr;  // pinned mirror of the control words
     float align_ms = 0.f, emit_ms = 0.f;
     bool timing = false;
@@ -99,6 +106,7 @@ struct ewfa_ctx {
         }
     } graph_key{};
     bool use_graph = true;
+    int policy_gen = 0, graph_policy_gen = -1;
     // host pipeline buffers
     DevBuf h_words, h_pairs, h_results, h_cigars;
     size_t resident_words = 0;
@@ -165,41 +173,69 @@ int finish_pass(ewfa_ctx* c, Pass& p, uint32_t items) {
     return EWFA_OK;
 }
 
-// Main pass of a call, from the policy's bucket for max_len.
-int main_pass(ewfa_ctx* c, uint32_t max_len, uint32_t items, bool persistent, Pass& p) {
+// Pass for pairs up to blen bases under one bucket's policy.
+int pass_from_bucket(ewfa_ctx* c, const ewfa::BucketPolicy& b, uint32_t blen, uint32_t items, Pass& p) {
     p = Pass{};
-    p.seq_words = seq_words_for(max_len);
-    p.persistent = persistent;
-    if (!persistent) {
-        // the reference's launch model: one workgroup of one wave per pair,
-        // a distance budget of 64 levels (src/wavefront.cuh:465)
-        p.waves = 1;
-        p.bt_lds = 1;
-        p.dcap = 64;
-        p.grab = 1;
-    } else {
-        const ewfa::BucketPolicy& b = c->policy.bucket_for(max_len);
-        p.waves = b.waves;
-        p.bt_lds = b.bt_lds;
-        p.grab = std::max(1, b.grab);
-        p.dcap = b.dcap;
-        // the bucket's dcap is a request; shrink it to what LDS holds
-        const int most = max_dcap_for(p.seq_words, p.bt_lds, lds_budget(c));
-        if (most < 0)
-            return fail(EWFA_ERR_RANGE, "sequences of %u bases do not fit in LDS", max_len);
-        if (p.dcap > most) p.dcap = most;
-        if (p.dcap < 1) p.dcap = 1;
-        // one wave with the backtrace in LDS: offsets in registers when they fit
-        if (p.waves == 1 && p.bt_lds) {
-            const int nc = ewfa::reg_chunks_for(p.dcap);
-            const int ow = overlapped_words_for(max_len);
-            if (nc > 0 && ewfa::lds_bytes_reg(ow, p.dcap) <= 65536) {
-                p.reg = nc;
-                p.seq_words = ow;
-            }
+    p.seq_words = seq_words_for(blen);
+    p.persistent = true;
+    p.waves = b.waves;
+    p.bt_lds = b.bt_lds;
+    p.grab = std::max(1, b.grab);
+    p.dcap = b.dcap;
+    // the bucket's dcap is a request; shrink it to what LDS holds
+    const int most = max_dcap_for(p.seq_words, p.bt_lds, lds_budget(c));
+    if (most < 0) return fail(EWFA_ERR_RANGE, "sequences of %u bases do not fit in LDS", blen);
+    if (p.dcap > most) p.dcap = most;
+    if (p.dcap < 1) p.dcap = 1;
+    // one wave with the backtrace in LDS: offsets in registers when they fit
+    if (p.waves == 1 && p.bt_lds) {
+        const int nc = ewfa::reg_chunks_for(p.dcap);
+        const int ow = overlapped_words_for(blen);
+        if (nc > 0 && ewfa::lds_bytes_reg(ow, p.dcap) <= 65536) {
+            p.reg = nc;
+            p.seq_words = ow;
         }
     }
     return finish_pass(c, p, items);
+}
+
+// Main pass of a plain (reference launch model) batch.
+int main_pass(ewfa_ctx* c, uint32_t max_len, uint32_t items, bool persistent, Pass& p) {
+    if (persistent) return pass_from_bucket(c, c->policy.bucket_for(max_len), max_len, items, p);
+    p = Pass{};
+    p.seq_words = seq_words_for(max_len);
+    p.persistent = false;
+    // the reference's launch model: one workgroup of one wave per pair, a
+    // distance budget of 64 levels (src/wavefront.cuh:465)
+    p.waves = 1;
+    p.bt_lds = 1;
+    p.dcap = 64;
+    p.grab = 1;
+    return finish_pass(c, p, items);
+}
+
+// The persistent path's plan: one pass per length bucket that pairs up to
+// max_len can fall in, run longest bucket first; a single bucket runs over
+// the identity list without the bucketing kernel.
+struct Plan {
+    int nb = 0;
+    ewfa::BucketTable table{};
+    Pass pass[ewfa::MAX_BUCKETS];
+    uint32_t blen[ewfa::MAX_BUCKETS];
+};
+
+int plan_persistent(ewfa_ctx* c, uint32_t max_len, uint32_t n, Plan& pl) {
+    pl = Plan{};
+    for (const ewfa::BucketPolicy& b : c->policy.buckets) {
+        if (pl.nb == ewfa::MAX_BUCKETS) break;
+        pl.table.hi[pl.nb] = b.len_hi;
+        pl.blen[pl.nb] = std::min(b.len_hi, max_len);
+        CHK(pass_from_bucket(c, b, pl.blen[pl.nb], n, pl.pass[pl.nb]));
+        ++pl.nb;
+        if (max_len <= b.len_hi) break;
+    }
+    pl.table.nb = pl.nb;
+    return EWFA_OK;
 }
 
 // Deferral pass after one with budget prev_dcap.
@@ -235,6 +271,20 @@ int reserve_bt(ewfa_ctx* c, const Pass& p) {
     if (p.bt_lds) return EWFA_OK;
     const size_t slot = 2u * (size_t)ewfa::bt_chunks((uint32_t)p.dcap);
     return c->bt.reserve(slot * p.grid * 8);
+}
+
+int reserve_plan(ewfa_ctx* c, const Plan& pl, size_t n) {
+    for (int b = 0; b < pl.nb; ++b) CHK(reserve_bt(c, pl.pass[b]));
+    if (pl.nb > 1) CHK(c->buckets.reserve((size_t)pl.nb * std::max<size_t>(n, 1) * 4));
+    return EWFA_OK;
+}
+
+int launch_pass(const ewfa::AlignArgs& a, const Pass& p, hipStream_t s) {
+    if (p.reg)
+        HIPCHK(ewfa::launch_align_reg(a, p.reg, p.persistent, p.grid, p.lds, s));
+    else
+        HIPCHK(ewfa::launch_align(a, p.waves, p.persistent, p.grid, p.lds, s));
+    return EWFA_OK;
 }
 
 ewfa::AlignArgs base_args(ewfa_ctx* c, const uint64_t* words, const ewfa_pair_t* pairs,
@@ -310,6 +360,48 @@ int enqueue_main(ewfa_ctx* c, const Pass& p, const uint64_t* words, const ewfa_p
     if (timing) HIPCHK(hipEventRecord(c->ev_e1, s));
     HIPCHK(hipMemcpyAsync(c->host_ctrl + CTRL_DEFER_A, ctrl + CTRL_DEFER_A, 4,
                           hipMemcpyDeviceToHost, s));
+    return EWFA_OK;
+}
+
+// Enqueues a persistent plan: bucketing, one align pass per bucket (longest
+// first), the emit per bucket, and the deferred count's copy to the host.
+int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_pair_t* pairs, uint32_t n,
+                 uint32_t max_len, ewfa_result_t* results, int8_t* cigars, hipStream_t s, bool timing) {
+    uint32_t* ctrl = c->ctrl.as<uint32_t>();
+    uint32_t* lists = c->buckets.as<uint32_t>();
+    const bool bucketed = pl.nb > 1;
+    HIPCHK(ewfa::launch_fill_u32(ctrl, 0u, CTRL_BCOUNT + ewfa::MAX_BUCKETS, s));
+    if (bucketed) HIPCHK(ewfa::launch_bucketize(pairs, n, pl.table, lists, ctrl + CTRL_BCOUNT, s));
+    if (timing) HIPCHK(hipEventRecord(c->ev_a0, s));
+    for (int b = pl.nb - 1; b >= 0; --b) {
+        const Pass& p = pl.pass[b];
+        ewfa::AlignArgs a = base_args(c, words, pairs, results, max_len);
+        fill_pass_args(a, p, c);
+        a.list = bucketed ? lists + (size_t)b * n : nullptr;
+        a.count = n;
+        a.count_ptr = bucketed ? ctrl + CTRL_BCOUNT + b : nullptr;
+        a.queue = ctrl + CTRL_BQUEUE + b;
+        a.defer_list = c->list_a.as<uint32_t>();
+        a.defer_count = ctrl + CTRL_DEFER_A;
+        a.max_len = pl.blen[b];
+        CHK(launch_pass(a, p, s));
+    }
+    if (timing) HIPCHK(hipEventRecord(c->ev_a1, s));
+    for (int b = pl.nb - 1; b >= 0; --b) {
+        ewfa::EmitArgs e{};
+        e.words = words;
+        e.pairs = pairs;
+        e.results = results;
+        e.ops = c->ops.as<uint64_t>();
+        e.ops_stride = ewfa::ops_words(max_len + 1);
+        e.cigars = cigars;
+        e.list = bucketed ? lists + (size_t)b * n : nullptr;
+        e.count = n;
+        e.count_ptr = bucketed ? ctrl + CTRL_BCOUNT + b : nullptr;
+        CHK(emit(c, e, n, pl.blen[b], true, s));
+    }
+    if (timing) HIPCHK(hipEventRecord(c->ev_e1, s));
+    HIPCHK(hipMemcpyAsync(c->host_ctrl + CTRL_DEFER_A, ctrl + CTRL_DEFER_A, 4, hipMemcpyDeviceToHost, s));
     return EWFA_OK;
 }
 
@@ -469,8 +561,8 @@ int ewfa_destroy(ewfa_ctx_t* c) {
     for (hipEvent_t e : {c->ev_in, c->ev_a0, c->ev_a1, c->ev_e1})
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
-    for (DevBuf* b : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->h_words, &c->h_pairs,
-                      &c->h_results, &c->h_cigars})
+    for (DevBuf* b : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->h_words,
+                      &c->h_pairs, &c->h_results, &c->h_cigars})
         b->release();
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
     delete c;
@@ -564,21 +656,20 @@ int ewfa_align_device(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t*
     c->align_ms = c->emit_ms = 0.f;
 
     if (c->opts.mode == EWFA_MODE_PERSISTENT) {
-        Pass p;
-        CHK(main_pass(c, max_len, (uint32_t)n, true, p));
-        CHK(reserve_bt(c, p));
+        Plan pl;
+        CHK(plan_persistent(c, max_len, (uint32_t)n, pl));
+        CHK(reserve_plan(c, pl, n));
         ewfa_ctx::Key key{d_words, d_pairs, d_results, d_cigars, n, max_len};
         const bool timing = c->timing;
         if (c->use_graph && !timing) {
-            if (!(c->graph && c->graph_key == key)) {
+            if (!(c->graph && c->graph_key == key && c->graph_policy_gen == c->policy_gen)) {
                 if (c->graph) {
                     (void)hipGraphExecDestroy(c->graph);
                     c->graph = nullptr;
                 }
                 hipGraph_t g = nullptr;
                 HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                int rc = enqueue_main(c, p, d_words, d_pairs, 0, (uint32_t)n, max_len, d_results,
-                                      d_cigars, s, false);
+                int rc = enqueue_plan(c, pl, d_words, d_pairs, (uint32_t)n, max_len, d_results, d_cigars, s, false);
                 hipError_t ce = hipStreamEndCapture(s, &g);
                 if (rc != EWFA_OK) {
                     if (g) (void)hipGraphDestroy(g);
@@ -592,6 +683,7 @@ int ewfa_align_device(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t*
                     return fail(EWFA_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ie));
                 }
                 c->graph_key = key;
+                c->graph_policy_gen = c->policy_gen;
             }
             HIPCHK(hipEventRecord(c->ev_a0, s));
             HIPCHK(hipGraphLaunch(c->graph, s));
@@ -600,8 +692,7 @@ int ewfa_align_device(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t*
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, c->ev_a0, c->ev_e1) == hipSuccess) c->align_ms = ms;
         } else {
-            CHK(enqueue_main(c, p, d_words, d_pairs, 0, (uint32_t)n, max_len, d_results, d_cigars,
-                             s, timing));
+            CHK(enqueue_plan(c, pl, d_words, d_pairs, (uint32_t)n, max_len, d_results, d_cigars, s, timing));
             HIPCHK(hipStreamSynchronize(s));
             if (timing) {
                 float ms = 0.f;
@@ -609,7 +700,9 @@ int ewfa_align_device(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t*
                 if (hipEventElapsedTime(&ms, c->ev_a1, c->ev_e1) == hipSuccess) c->emit_ms = ms;
             }
         }
-        CHK(resolve_deferred(c, p.dcap, d_words, d_pairs, 0, max_len, d_results, d_cigars, s));
+        int dcap_min = pl.pass[0].dcap;
+        for (int b = 1; b < pl.nb; ++b) dcap_min = std::min(dcap_min, pl.pass[b].dcap);
+        CHK(resolve_deferred(c, dcap_min, d_words, d_pairs, 0, max_len, d_results, d_cigars, s));
         return EWFA_OK;
     }
 

@@ -39,6 +39,12 @@ struct BucketPolicy {
 
 constexpr int MAX_BUCKETS = 8;
 
+// Length limits of the buckets, by value into the bucketing kernel.
+struct BucketTable {
+    uint32_t hi[MAX_BUCKETS];
+    int nb;
+};
+
 // Arguments of the align kernel (passed by value).
 struct AlignArgs {
     const uint64_t* words;
@@ -114,6 +120,8 @@ constexpr int EMIT_LDS_MAX_SW = 64;  // 64 KiB of LDS per block
 hipError_t launch_pack(const char* ascii, size_t ascii_stride, const uint32_t* len, size_t nseq,
                        uint64_t* words, size_t words_stride, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
+hipError_t launch_bucketize(const ewfa_pair_t* pairs, uint32_t n, const BucketTable& t, uint32_t* lists,
+                            uint32_t* counts, hipStream_t stream);
 // Occupancy of the align kernel for a configuration (blocks per CU).
 int align_blocks_per_cu(int waves, size_t lds);
 hipError_t align_set_max_lds(int waves, size_t lds);

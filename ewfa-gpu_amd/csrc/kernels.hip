@@ -208,8 +208,14 @@ __device__ __forceinline__ int slot_bt_word(int seq_words, int dcap) {
     return 2 * seq_words + ((2 * dcap + 5 + 1) & ~1);
 }
 
-// End of a pair's search: status, deferral, traceback into the ops scratch.
-// Called by every thread of the group with the same (done, d, limit).
+// End of a pair's search: status, deferral, and the traceback into the ops
+// scratch.  Called by every thread of the group with the same arguments; the
+// traceback runs on the group's first wave.  For each run of 64 levels, lane
+// j loads the backtrace chunks level top-j can need: the path diagonal moves
+// by at most one per level, so there it lies within j of the diagonal at level
+// top, i.e. in at most three consecutive 64-diagonal chunks.  The walk then
+// takes each op bit out of lane j with readlane (scalar work), from LDS or
+// HBM alike.
 __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, const uint64_t* BT, int gtid,
                                             bool done, int d, int limit, int tk) {
     ewfa_result_t* res = a.results + pidx;
@@ -233,66 +239,8 @@ __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, c
             return;
         }
     }
-
-    if (gtid == 0) {
-        // traceback: op of (lev, k) selects the predecessor diagonal k+op-2
-        uint64_t* ow = a.ops + (size_t)pidx * a.ops_stride;
-        if (dist > 0) {
-            int wi = (dist - 1) >> 5;
-            uint64_t word = 0;
-            int k = tk;
-            const bool trace = tka <= start;
-            for (int lev = start; trace && lev >= 1; --lev) {
-                const int s = lev - 1;
-                while (wi > (s >> 5)) {
-                    ow[wi--] = word;
-                    word = 0;
-                }
-                const int idx = k + lev;
-                if (idx < 0 || idx > 2 * lev) break;  // cannot happen for a valid table
-                const uint64_t* q = BT + 2 * (size_t)(bt_level_base((uint32_t)lev) + (uint32_t)(idx >> 6));
-                const int bit = idx & 63;
-                const int op = (int)((q[0] >> bit) & 1ull) | ((int)((q[1] >> bit) & 1ull) << 1);
-                word |= (uint64_t)op << ((s & 31) << 1);
-                k += op - 2;
-            }
-            while (wi >= 0) {
-                ow[wi--] = word;
-                word = 0;
-            }
-        }
-        *res = ewfa_result_t{dist, 0, status, 0};
-    }
-}
-
-// finish_pair for one wave whose backtrace rows hold at most NC chunks per
-// level: the 64 lanes load the rows of 64 consecutive levels at once (lane j
-// holds level top-j), then the walk takes each op bit out of lane j with
-// readlane, so the serial part runs on scalar registers.
-template <int NC>
-__device__ __forceinline__ void finish_pair_rows(const AlignArgs& a, uint32_t pidx, const uint64_t* BT, int lane,
-                                                 bool done, int d, int limit, int tk) {
-    ewfa_result_t* res = a.results + pidx;
-    const int tka = tk < 0 ? -tk : tk;
-    const int cap = a.max_distance;
-    int status = EWFA_PAIR_OK;
-    int dist = d, start = d;
-    if (!done) {
-        if (cap > 0 && limit == cap - 1) {
-            // the reference's cap: distance = cap, ops of level cap-1 at the
-            // target diagonal (src/kernels.cu:259, :288-303)
-            status = tka > cap ? EWFA_PAIR_UNDEFINED : EWFA_PAIR_CAPPED;
-            dist = cap;
-            start = cap - 1;
-        } else {
-            if (lane == 0) {
-                const uint32_t at = atomicAdd(a.defer_count, 1u);
-                a.defer_list[at] = pidx;
-                res->status = PAIR_DEFERRED;
-            }
-            return;
-        }
-    }
+    if (gtid >= 64) return;
+    const int lane = gtid;
     uint64_t* ow = a.ops + (size_t)pidx * a.ops_stride;
     if (dist > 0) {
         int wi = (dist - 1) >> 5;
@@ -301,20 +249,18 @@ __device__ __forceinline__ void finish_pair_rows(const AlignArgs& a, uint32_t pi
         const int last = tka <= start ? 1 : start + 1;  // no walk when |t| > start
         for (int top = start; top >= last; top -= 64) {
             const int lev_l = top - lane;
-            uint64_t rlo[NC], rhi[NC];
+            const int c0 = max(0, (k + top - 2 * lane) >> 6);
+            uint64_t r[3][2];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                rlo[c] = 0;
-                rhi[c] = 0;
-            }
+            for (int u = 0; u < 3; ++u) r[u][0] = r[u][1] = 0;
             if (lev_l >= 1) {
                 const uint64_t* row = BT + 2 * (size_t)bt_level_base((uint32_t)lev_l);
                 const int nch = (lev_l >> 5) + 1;
 #pragma unroll
-                for (int c = 0; c < NC; ++c)
-                    if (c < nch) {
-                        rlo[c] = row[2 * c];
-                        rhi[c] = row[2 * c + 1];
+                for (int u = 0; u < 3; ++u)
+                    if (c0 + u < nch) {
+                        r[u][0] = row[2 * (c0 + u)];
+                        r[u][1] = row[2 * (c0 + u) + 1];
                     }
             }
             const int steps = min(64, top - last + 1);
@@ -327,18 +273,18 @@ __device__ __forceinline__ void finish_pair_rows(const AlignArgs& a, uint32_t pi
                     word = 0;
                 }
                 const int idx = k + lev;
-                const int c = idx >> 6, bit = idx & 63;
+                const int rel = (idx >> 6) - __builtin_amdgcn_readlane(c0, j);
+                const int bit = idx & 63;
                 uint32_t wl = 0, wh = 0;
 #pragma unroll
-                for (int cc = 0; cc < NC; ++cc)
-                    if (cc == c) {
-                        const uint64_t l64 = rlo[cc], h64 = rhi[cc];
+                for (int u = 0; u < 3; ++u)
+                    if (rel == u) {
                         if (bit >= 32) {
-                            wl = __builtin_amdgcn_readlane((int)(uint32_t)(l64 >> 32), j);
-                            wh = __builtin_amdgcn_readlane((int)(uint32_t)(h64 >> 32), j);
+                            wl = __builtin_amdgcn_readlane((int)(uint32_t)(r[u][0] >> 32), j);
+                            wh = __builtin_amdgcn_readlane((int)(uint32_t)(r[u][1] >> 32), j);
                         } else {
-                            wl = __builtin_amdgcn_readlane((int)(uint32_t)l64, j);
-                            wh = __builtin_amdgcn_readlane((int)(uint32_t)h64, j);
+                            wl = __builtin_amdgcn_readlane((int)(uint32_t)r[u][0], j);
+                            wh = __builtin_amdgcn_readlane((int)(uint32_t)r[u][1], j);
                         }
                     }
                 const int b = bit & 31;
@@ -619,7 +565,7 @@ __device__ __forceinline__ void align_pair_reg(const AlignArgs& a, uint32_t pidx
         }
     }
     group_sync<1>();
-    finish_pair_rows<NC>(a, pidx, BT, lane, done, d, limit, tk);
+    finish_pair(a, pidx, BT, lane, done, d, limit, tk);
 }
 
 template <int NC>
@@ -898,6 +844,34 @@ __global__ __launch_bounds__(64) void emit_lds_kernel(EmitArgs a, int sw) {
 }
 
 // ---------------------------------------------------------------------------
+// length buckets
+
+// Sorts pair indices into per-bucket lists by max(plen, tlen): bucket b takes
+// lengths up to hi[b] (the last bucket takes the rest).  One atomic per bucket
+// per wave; the order inside a bucket does not matter for the results.
+__global__ __launch_bounds__(256) void bucketize_kernel(const ewfa_pair_t* pairs, uint32_t n, BucketTable t,
+                                                        uint32_t* lists, uint32_t* counts) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int b = -1;
+    if (i < n) {
+        const uint32_t L = max(pairs[i].plen, pairs[i].tlen);
+        b = t.nb - 1;
+        for (int q = t.nb - 2; q >= 0; --q)
+            if (L <= t.hi[q]) b = q;
+    }
+    for (int q = 0; q < t.nb; ++q) {
+        const uint64_t m = __ballot(b == q);
+        if (m == 0) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(counts + q, (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, leader);
+        if (b == q) lists[(size_t)q * n + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // pack and fill
 
 __global__ __launch_bounds__(256) void pack_kernel(const char* ascii, size_t ascii_stride,
@@ -1036,6 +1010,13 @@ hipError_t launch_pack(const char* ascii, size_t ascii_stride, const uint32_t* l
     if (total == 0) return hipSuccess;
     hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
                        ascii, ascii_stride, len, nseq, words, words_stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_bucketize(const ewfa_pair_t* pairs, uint32_t n, const BucketTable& t, uint32_t* lists,
+                            uint32_t* counts, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(bucketize_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, pairs, n, t, lists, counts);
     return hipGetLastError();
 }
 

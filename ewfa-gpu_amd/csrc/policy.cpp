@@ -19,7 +19,6 @@ const BucketPolicy& Policy::bucket_for(uint32_t len) const {
 Policy default_policy() {
     Policy p;
     //              len_hi   waves dcap  bt_lds grab
-    p.buckets.push_back({512u, 1, 63, 1, 16});
     p.buckets.push_back({1200u, 1, 63, 1, 16});
     p.buckets.push_back({2048u, 1, 127, 1, 8});
     p.buckets.push_back({8192u, 4, 512, 0, 1});
