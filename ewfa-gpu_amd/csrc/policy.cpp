@@ -22,7 +22,7 @@ Policy default_policy() {
     p.buckets.push_back({1200u, 1, 63, 1, 16});
     p.buckets.push_back({2048u, 1, 127, 1, 8});
     p.buckets.push_back({8192u, 4, 512, 0, 1});
-    p.buckets.push_back({0xFFFFFFFFu, 8, 2048, 0, 1});
+    p.buckets.push_back({0xFFFFFFFFu, 16, 1 << 20, 0, 1});  // as many levels as LDS holds
     return p;
 }
 

@@ -50,6 +50,17 @@ class CliArgs(ctypes.Structure):
                 ("plain", ctypes.c_int)]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("arena_bytes", ctypes.c_size_t),
+                ("max_distance", ctypes.c_int), ("policy_path", ctypes.c_char_p)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("arena_bytes", ctypes.c_size_t), ("arena_peak", ctypes.c_size_t), ("slots", ctypes.c_int),
+                ("peak_running", ctypes.c_int)]
+
+
 class PairsFile(ctypes.Structure):
     _fields_ = [("n", ctypes.c_size_t), ("n_words", ctypes.c_size_t), ("words", ctypes.POINTER(ctypes.c_uint64)),
                 ("pairs", ctypes.c_void_p), ("ascii", ctypes.POINTER(ctypes.c_char)),
@@ -65,6 +76,8 @@ EXPORTS = [
     "ewfa_cigar_text_standard", "ewfa_check_cigar", "ewfa_read_pairs", "ewfa_free_pairs",
     "ewfa_synth_max_len", "ewfa_synth_pairs", "ewfa_cli_parse", "ewfa_cli_usage",
     "ewfa_device_alloc", "ewfa_device_free", "ewfa_memcpy", "ewfa_device_synchronize",
+    "ewfa_server_create", "ewfa_server_destroy", "ewfa_server_submit", "ewfa_server_poll", "ewfa_server_wait",
+    "ewfa_server_job_latency", "ewfa_server_stats",
 ]
 
 _lib = None
@@ -112,6 +125,13 @@ def lib():
     L.ewfa_device_free.argtypes = [_vp]
     L.ewfa_memcpy.argtypes = [_vp, _vp, _sz, ctypes.c_int]
     L.ewfa_device_synchronize.argtypes = [ctypes.c_int]
+    L.ewfa_server_create.argtypes = [ctypes.POINTER(_vp), ctypes.POINTER(ServerOpts)]
+    L.ewfa_server_destroy.argtypes = [_vp]
+    L.ewfa_server_submit.argtypes = [_vp, _vp, _sz, _vp, _sz, _vp, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)]
+    L.ewfa_server_poll.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]
+    L.ewfa_server_wait.argtypes = [_vp, ctypes.c_uint64]
+    L.ewfa_server_job_latency.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+    L.ewfa_server_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
     _lib = L
     return L
 
@@ -370,3 +390,68 @@ class Context:
         r = lib().ewfa_max_distance(self.handle, max_len)
         _check(r if r < 0 else OK)
         return r
+
+
+# ---------------------------------------------------------------------------
+# serving
+
+class ServerJob:
+    """A submitted job: owns its host buffers until it completes."""
+
+    def __init__(self, server, words, pairs, n, cigar_bytes):
+        self.server = server
+        self.words = np.ascontiguousarray(words)
+        self.pairs = np.ascontiguousarray(pairs[:max(n, 1)])
+        self.n = n
+        self.results = np.zeros(max(n, 1), dtype=RESULT_DTYPE)
+        self.cigars = np.zeros(max(cigar_bytes, 4), dtype=np.int8)
+        jid = ctypes.c_uint64()
+        _check(lib().ewfa_server_submit(server.handle, self.words.ctypes.data, self.words.size,
+                                        self.pairs.ctypes.data, n, self.results.ctypes.data,
+                                        self.cigars.ctypes.data, self.cigars.size, ctypes.byref(jid)))
+        self.id = jid.value
+
+    def done(self):
+        d = ctypes.c_int()
+        _check(lib().ewfa_server_poll(self.server.handle, self.id, ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self):
+        _check(lib().ewfa_server_wait(self.server.handle, self.id))
+        return self.results[: self.n], self.cigars
+
+    def latency(self):
+        s = ctypes.c_double()
+        _check(lib().ewfa_server_job_latency(self.server.handle, self.id, ctypes.byref(s)))
+        return s.value
+
+
+class Server:
+    """The job server (continuous admission by arena footprint)."""
+
+    def __init__(self, device=0, slots=4, arena_bytes=0, max_distance=0, policy_path=None):
+        self._policy = policy_path.encode() if policy_path else None
+        o = ServerOpts(device=device, slots=slots, arena_bytes=arena_bytes, max_distance=max_distance,
+                       policy_path=self._policy)
+        h = _vp()
+        _check(lib().ewfa_server_create(ctypes.byref(h), ctypes.byref(o)))
+        self.handle = h
+
+    def submit(self, words, pairs, n, cigar_bytes):
+        return ServerJob(self, words, pairs, n, cigar_bytes)
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().ewfa_server_stats(self.handle, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().ewfa_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

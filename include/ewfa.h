@@ -171,6 +171,50 @@ int ewfa_set_flags(ewfa_ctx_t *ctx, int flags);
 /* Largest distance the device path can resolve for sequences up to max_len. */
 int ewfa_max_distance(ewfa_ctx_t *ctx, uint32_t max_len);
 
+/* ---- serving -------------------------------------------------------------- */
+
+/* A job server (an addition; the reference has no serving path).  Jobs are
+ * admitted first-come first-served when a slot (a device context with its own
+ * stream and worker thread) is free and the device arena has room for the
+ * job's footprint: its packed sequences, descriptors, results and CIGAR
+ * streams, which stay resident on the device while it runs, plus its op
+ * scratch.  Admitted jobs run concurrently; later arrivals wait in the queue.
+ * Every job's results are identical to aligning it alone. */
+typedef struct ewfa_server ewfa_server_t;
+
+typedef struct ewfa_server_opts {
+    int device;
+    int slots;               /* concurrent jobs (0: 4) */
+    size_t arena_bytes;      /* device arena for resident job data (0: 4 GiB) */
+    int max_distance;        /* as ewfa_opts_t */
+    const char *policy_path; /* as ewfa_opts_t */
+} ewfa_server_opts_t;
+
+typedef struct ewfa_server_stats {
+    uint64_t submitted, completed, failed;
+    size_t arena_bytes, arena_peak;   /* capacity and peak use */
+    int slots, peak_running;          /* slots and peak concurrent jobs */
+} ewfa_server_stats_t;
+
+int ewfa_server_create(ewfa_server_t **srv, const ewfa_server_opts_t *opts);
+/* Waits for every submitted job, then frees the server. */
+int ewfa_server_destroy(ewfa_server_t *srv);
+
+/* Queues a job of n pairs laid out as for ewfa_align_host; the host buffers
+ * must stay valid until the job completes.  Returns at once with its id.
+ * EWFA_ERR_RANGE when the job can never fit the arena. */
+int ewfa_server_submit(ewfa_server_t *srv, const uint64_t *h_words, size_t n_words,
+                       const ewfa_pair_t *h_pairs, size_t n, ewfa_result_t *h_results,
+                       int8_t *h_cigars, size_t cigar_bytes, uint64_t *job_id);
+
+/* *done = 1 once the job's results are in its host buffers. */
+int ewfa_server_poll(ewfa_server_t *srv, uint64_t job_id, int *done);
+/* Blocks until the job completes; returns the job's own status. */
+int ewfa_server_wait(ewfa_server_t *srv, uint64_t job_id);
+/* Seconds from submit to completion of a finished job (host clock). */
+int ewfa_server_job_latency(ewfa_server_t *srv, uint64_t job_id, double *seconds);
+int ewfa_server_stats(ewfa_server_t *srv, ewfa_server_stats_t *out);
+
 /* ---- host helpers (CPU, no device needed) ------------------------------- */
 
 /* ASCII -> packed words on the host (same layout as ewfa_pack_ascii). */

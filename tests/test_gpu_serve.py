@@ -1,0 +1,79 @@
+"""The job server: every job's results are identical alone and in the mix
+(tolerance 0: the work is integer), jobs overlap on the device, and an arena
+smaller than the offered load still admits and completes every job."""
+import numpy as np
+import pytest
+
+import ewfa
+import oracle_py as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_jobs(count, seed=5, max_pairs=300):
+    rng = np.random.default_rng(seed)
+    spec = ewfa.synth_spec(kind=1, seed=seed, len_lo=100, len_hi=3000, err_lo=0.01, err_hi=0.10)
+    jobs = []
+    first = 0
+    for _ in range(count):
+        n = int(rng.integers(1, max_pairs + 1))
+        w = ewfa.Workload(spec, n, first=first, threads=4)
+        first += n
+        jobs.append(w)
+    return jobs
+
+
+def alone(w):
+    with ewfa.Context() as ctx:
+        return ctx.align_host(w.words, w.pairs, w.n, w.cigar_bytes)
+
+
+def test_each_job_same_alone_and_in_the_mix():
+    jobs = make_jobs(24)
+    say(f"case serve mix: {len(jobs)} jobs, {sum(w.n for w in jobs)} pairs, 4 slots")
+    ref = [alone(w) for w in jobs]
+    with ewfa.Server(slots=4) as srv:
+        handles = [srv.submit(w.words, w.pairs, w.n, w.cigar_bytes) for w in jobs]
+        outs = [h.wait() for h in handles]
+        lat = [h.latency() for h in handles]
+        st = srv.stats()
+    assert st["completed"] == len(jobs) and st["failed"] == 0
+    assert st["peak_running"] > 1  # admitted jobs ran side by side
+    for w, (r0, c0), (r1, c1) in zip(jobs, ref, outs):
+        assert np.array_equal(r0, r1)
+        s0 = ewfa.streams_from(r0, w.pairs, c0)
+        s1 = ewfa.streams_from(r1, w.pairs, c1)
+        assert all(np.array_equal(a, b) for a, b in zip(s0, s1))
+    assert all(x > 0 for x in lat)
+    # and the mix is right, not just consistent: spot-check against the oracle
+    w = jobs[3]
+    ok, d, sl, streams = orc.batch_workload(w)
+    got = ewfa.streams_from(outs[3][0], w.pairs, outs[3][1])
+    assert all(np.array_equal(a, b) for a, b in zip(got, streams))
+
+
+def test_small_arena_admits_as_memory_frees():
+    jobs = make_jobs(12, seed=6, max_pairs=200)
+    need = max(w.n_words * 8 + w.n * 56 + w.cigar_bytes for w in jobs)
+    arena = int(need * 2.5)  # room for about two jobs at a time
+    say(f"case serve arena {arena} bytes, {len(jobs)} jobs")
+    with ewfa.Server(slots=4, arena_bytes=arena) as srv:
+        handles = [srv.submit(w.words, w.pairs, w.n, w.cigar_bytes) for w in jobs]
+        outs = [h.wait() for h in handles]
+        st = srv.stats()
+    assert st["completed"] == len(jobs) and st["failed"] == 0
+    assert st["arena_peak"] <= arena
+    for w, (r, c) in zip(jobs, outs):
+        r0, c0 = alone(w)
+        assert np.array_equal(r0, r)
+
+
+def test_job_larger_than_arena_is_refused():
+    w = make_jobs(1, seed=7, max_pairs=50)[0]
+    with ewfa.Server(slots=1, arena_bytes=1 << 12) as srv:
+        with pytest.raises(ewfa.EwfaError):
+            srv.submit(w.words, w.pairs, w.n, w.cigar_bytes)
