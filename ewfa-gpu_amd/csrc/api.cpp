@@ -469,6 +469,16 @@ int resolve_deferred(ewfa_ctx* c, int prev_dcap, const uint64_t* words, const ew
 
 }  // namespace
 
+namespace ewfa {
+int set_context_policy(ewfa_ctx_t* c, const Policy& p) {
+    if (!c || p.buckets.empty()) return fail(EWFA_ERR_ARG, "empty policy");
+    c->policy = p;
+    c->policy.buckets.back().len_hi = 0xFFFFFFFFu;
+    ++c->policy_gen;
+    return EWFA_OK;
+}
+}  // namespace ewfa
+
 extern "C" {
 
 void ewfa_default_opts(ewfa_opts_t* o) {
@@ -598,6 +608,21 @@ int ewfa_memcpy(void* dst, const void* src, size_t bytes, int kind) {
         default: return fail(EWFA_ERR_ARG, "unknown copy kind %d", kind);
     }
     HIPCHK(hipMemcpy(dst, src, bytes, k));
+    return EWFA_OK;
+}
+
+int ewfa_host_alloc(size_t bytes, void** ptr) {
+    if (!ptr) return fail(EWFA_ERR_ARG, "ptr is NULL");
+    *ptr = nullptr;
+    if (hipHostMalloc(ptr, bytes ? bytes : 4, hipHostMallocDefault) != hipSuccess) {
+        *ptr = nullptr;
+        return fail(EWFA_ERR_NOMEM, "hipHostMalloc(%zu) failed", bytes);
+    }
+    return EWFA_OK;
+}
+
+int ewfa_host_free(void* ptr) {
+    if (ptr) HIPCHK(hipHostFree(ptr));
     return EWFA_OK;
 }
 

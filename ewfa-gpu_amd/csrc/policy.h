@@ -27,4 +27,7 @@ int load_policy(const char* path, Policy& out);
 // Writes a policy file.
 int save_policy(const char* path, const Policy& p);
 
+// Replaces a context's policy (the tuner's candidates).
+int set_context_policy(ewfa_ctx_t* ctx, const Policy& p);
+
 }  // namespace ewfa

@@ -77,7 +77,7 @@ EXPORTS = [
     "ewfa_synth_max_len", "ewfa_synth_pairs", "ewfa_cli_parse", "ewfa_cli_usage",
     "ewfa_device_alloc", "ewfa_device_free", "ewfa_memcpy", "ewfa_device_synchronize",
     "ewfa_server_create", "ewfa_server_destroy", "ewfa_server_submit", "ewfa_server_poll", "ewfa_server_wait",
-    "ewfa_server_job_latency", "ewfa_server_stats",
+    "ewfa_server_job_latency", "ewfa_server_stats", "ewfa_autotune", "ewfa_host_alloc", "ewfa_host_free",
 ]
 
 _lib = None
@@ -125,6 +125,8 @@ def lib():
     L.ewfa_device_free.argtypes = [_vp]
     L.ewfa_memcpy.argtypes = [_vp, _vp, _sz, ctypes.c_int]
     L.ewfa_device_synchronize.argtypes = [ctypes.c_int]
+    L.ewfa_host_alloc.argtypes = [_sz, ctypes.POINTER(_vp)]
+    L.ewfa_host_free.argtypes = [_vp]
     L.ewfa_server_create.argtypes = [ctypes.POINTER(_vp), ctypes.POINTER(ServerOpts)]
     L.ewfa_server_destroy.argtypes = [_vp]
     L.ewfa_server_submit.argtypes = [_vp, _vp, _sz, _vp, _sz, _vp, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)]
@@ -132,6 +134,8 @@ def lib():
     L.ewfa_server_wait.argtypes = [_vp, ctypes.c_uint64]
     L.ewfa_server_job_latency.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
     L.ewfa_server_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
+    L.ewfa_autotune.argtypes = [ctypes.c_int, _vp, _sz, _vp, _sz, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                ctypes.c_char_p, _sz]
     _lib = L
     return L
 
@@ -329,6 +333,26 @@ class DeviceBuffer:
             pass
 
 
+class PinnedArray:
+    """A numpy view of pinned host memory (freed with the object)."""
+
+    def __init__(self, count, dtype):
+        dtype = np.dtype(dtype)
+        p = _vp()
+        _check(lib().ewfa_host_alloc(max(count * dtype.itemsize, 4), ctypes.byref(p)))
+        self.ptr = p.value
+        buf = (ctypes.c_uint8 * max(count * dtype.itemsize, 4)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=np.uint8)[: count * dtype.itemsize].view(dtype)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().ewfa_host_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
 def synchronize(device=0):
     _check(lib().ewfa_device_synchronize(device))
 
@@ -455,3 +479,18 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+# ---------------------------------------------------------------------------
+# autotuner
+
+def autotune(words, pairs, n, bucket_hi, out_path, reps=2, device=0):
+    """Times candidate configurations per length bucket on a sample and
+    writes the fastest to out_path; returns the text report."""
+    hi = np.ascontiguousarray(bucket_hi, dtype=np.uint32)
+    rep = ctypes.create_string_buffer(1 << 16)
+    words = np.ascontiguousarray(words)
+    pairs = np.ascontiguousarray(pairs[:max(n, 1)])
+    _check(lib().ewfa_autotune(device, words.ctypes.data, words.size, pairs.ctypes.data, n, hi.ctypes.data,
+                               len(hi), reps, out_path.encode(), rep, len(rep)))
+    return rep.value.decode()

@@ -123,6 +123,10 @@ int ewfa_device_free(void *ptr);
 int ewfa_memcpy(void *dst, const void *src, size_t bytes, int kind);
 int ewfa_device_synchronize(int device);
 
+/* Pinned (page-locked) host memory for fast copies. */
+int ewfa_host_alloc(size_t bytes, void **ptr);
+int ewfa_host_free(void *ptr);
+
 /* ASCII -> packed words on the device.  Replaces compact_sequences
  * (src/kernels.cu:313-394).  Sequence j is read from d_ascii + j*ascii_stride,
  * d_len[j] bases, and written to d_words + j*words_stride (words_stride >=
@@ -214,6 +218,22 @@ int ewfa_server_wait(ewfa_server_t *srv, uint64_t job_id);
 /* Seconds from submit to completion of a finished job (host clock). */
 int ewfa_server_job_latency(ewfa_server_t *srv, uint64_t job_id, double *seconds);
 int ewfa_server_stats(ewfa_server_t *srv, ewfa_server_stats_t *out);
+
+/* ---- autotuner ------------------------------------------------------------ */
+
+/* Times candidate launch configurations for each length bucket on the pairs
+ * of a sample (host buffers, laid out as for ewfa_align_host) and writes the
+ * fastest per bucket to a policy file for ewfa_opts_t.policy_path.  Bucket b
+ * takes pairs with max(plen, tlen) <= bucket_hi[b]; the last takes the rest.
+ * Candidates: one wave per pair with the wavefront in registers or LDS, and
+ * 4/8/16 waves per pair with the backtrace in HBM, over several wavefront
+ * budgets and queue grab sizes.  Every candidate must give byte-identical
+ * results and streams (a policy changes speed only): a difference fails the
+ * call with EWFA_ERR_STATE.  A text report of every timing goes to report
+ * (may be NULL).  An addition; the reference has no tuner. */
+int ewfa_autotune(int device, const uint64_t *h_words, size_t n_words, const ewfa_pair_t *h_pairs,
+                  size_t n, const uint32_t *bucket_hi, int nb, int reps, const char *out_path,
+                  char *report, size_t report_cap);
 
 /* ---- host helpers (CPU, no device needed) ------------------------------- */
 
