@@ -1,19 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark of the WFA edit aligner on one MI355X.
 
-Workload: 1,000,000 pairs of 1,000 bp, uniform ACGT, 5% edits split equally
-between substitution, insertion and deletion (seed 2), 2-bit packed uint64
-input resident in HBM, the reference's per-pair CIGAR buffer of 1,004 bytes.
-One step = distances and full CIGAR streams for all 1,000,000 pairs.
+Headline workload (the JSON `value`): 1,000,000 pairs of 1,000 bp, uniform
+ACGT, 5% edits split equally between substitution, insertion and deletion
+(seed 2), 2-bit packed uint64 input resident in HBM, the reference's
+per-pair CIGAR buffer of 1,004 bytes.  One step = distances and full CIGAR
+streams for all 1,000,000 pairs on the persistent path (one hipGraph-launched
+persistent kernel over a device work queue, then the emit kernel), under the
+launch policy the autotuner wrote for this workload.
 
-value      : alignments/s of the persistent path (one hipGraph-launched
-             persistent kernel over a device work queue, then the emit kernel)
-plain      : alignments/s of the reference's launch model (batches of 100,000,
-             one workgroup per pair, synchronised per batch) on the same input
-roofline   : algorithmic bytes of the path per launch / average duration of
-             the align kernel (HIP events on its launch stream)
-cpu_baseline: the oracle (a C restatement of the reference's algorithm with
-             its CPU validity check) on a bounded sample of the same workload
+Beside it, on the same box and run:
+  plain      the reference's launch model (batches of 100,000, one workgroup
+             per pair, synchronised per batch): kernels over resident input,
+             and end to end (per batch: packed words and descriptors host to
+             device, align, emit, results and CIGAR buffers back)
+  c3         500,000 pairs, log-uniform 100..10,000 bp, 1..10% edits, one
+             launch under the autotuned per-bucket policy
+  c4         20,000 pairs of 10..50 kbp at 10..15% edits with indel runs
+             (backtrace spilled to HBM), with the device footprint
+  served     Poisson arrivals of jobs of 1..4,096 pairs (lengths and errors as
+             c3) into the job server; p50/p99 latency from arrival to results
+             in host memory, and alignments/s against the c3 batched rate
+  roofline   algorithmic bytes per align launch / its average duration from
+             HIP events recorded (inside the graph) on its launch stream
+  cpu        the oracle (a C restatement of the reference's algorithm with its
+             CPU validity check) on a bounded sample, and on c1
 """
 import argparse
 import json
@@ -27,10 +38,331 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "alignments/s with CIGAR @1000bp/5% err (batched vs plain); served p50/p99"
 HBM_PEAK_GBS = 8000.0
+POLICY_C2 = os.path.join(ROOT, "ewfa-gpu_amd", "policy", "c2_1000bp_5pct.policy")
+POLICY_C3 = os.path.join(ROOT, "ewfa-gpu_amd", "policy", "c3_mixed_100_10000bp.policy")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def algo_bytes(w, res):
+    """Algorithmic bytes: 2-bit inputs, two u32 lengths, the compressed
+    CIGAR stream per pair."""
+    import numpy as np
+    plen = w.plen.astype(np.int64)
+    tlen = w.tlen.astype(np.int64)
+    clen = res["cigar_len"].astype(np.int64)
+    return int(((plen + 3) // 4 + (tlen + 3) // 4 + 8 + clen).sum())
+
+
+def cells(res):
+    import numpy as np
+    d = res["distance"].astype(np.int64)
+    return int(((d + 1) ** 2).sum())
+
+
+class Resident:
+    """A workload's inputs and outputs in device memory."""
+
+    def __init__(self, w):
+        import ewfa
+        self.w = w
+        self.dw = ewfa.DeviceBuffer.from_array(w.words)
+        self.dp = ewfa.DeviceBuffer.from_array(w.pairs[:w.n])
+        self.dr = ewfa.DeviceBuffer(w.n * 16)
+        self.dc = ewfa.DeviceBuffer(w.cigar_bytes)
+
+    def run(self, ctx):
+        ctx.align_device(self.dw.ptr, self.dp.ptr, self.w.n, self.w.max_len, self.dr.ptr, self.dc.ptr, None)
+
+    def results(self):
+        import ewfa
+        return self.dr.download(ewfa.RESULT_DTYPE, self.w.n)
+
+    def cigars(self):
+        import numpy as np
+        return self.dc.download(np.int8)
+
+    def nbytes(self):
+        return self.dw.nbytes + self.dp.nbytes + self.dr.nbytes + self.dc.nbytes
+
+
+def timed(run, ctx, steps, warmup):
+    """Warm-up, then `steps` timed calls bracketed by device synchronisation;
+    returns (seconds, [align ms], [emit ms]) with the per-call kernel times."""
+    import ewfa
+    for _ in range(warmup):
+        run(ctx)
+    ewfa.synchronize()
+    a_ms, e_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run(ctx)
+        a, e = ctx.last_timing()
+        a_ms.append(a)
+        e_ms.append(e)
+    ewfa.synchronize()
+    return time.perf_counter() - t0, a_ms, e_ms
+
+
+def leg_c2(args, threads):
+    import numpy as np
+    import ewfa
+    n = args.pairs
+    t0 = time.time()
+    w = ewfa.Workload(ewfa.synth_spec(**ewfa.CONFIGS["c2"]), n, cigar_cap=1004, threads=threads)
+    log(f"c2: generated {n} pairs in {time.time() - t0:.1f}s, max_len {w.max_len}")
+    r = Resident(w)
+    out = {}
+
+    # the value: persistent path, hipGraph replay, timing events in the graph
+    ctx = ewfa.Context(mode=ewfa.MODE_PERSISTENT, flags=ewfa.FLAG_TIMING, policy_path=POLICY_C2)
+    dt, a_ms, e_ms = timed(r.run, ctx, args.steps, args.warmup)
+    out["value"] = args.steps * n / dt
+    out["dt"] = dt
+    out["align_ms"] = float(np.mean(a_ms))
+    out["emit_ms"] = float(np.mean(e_ms))
+    log(f"c2 persistent: {args.steps} steps in {dt:.3f}s -> {out['value'] / 1e6:.2f} M aln/s; "
+        f"align {out['align_ms']:.3f} ms, emit {out['emit_ms']:.3f} ms per step")
+    res = r.results()
+    cig = r.cigars()
+    out["res"], out["cig"], out["w"] = res, cig, w
+    out["algo_bytes"] = algo_bytes(w, res)
+    out["cells"] = cells(res)
+
+    # plain: the reference's launch model over the resident input
+    ctx_p = ewfa.Context(mode=ewfa.MODE_PLAIN, batch_size=args.batch, flags=ewfa.FLAG_TIMING)
+    pdt, pa, pe = timed(r.run, ctx_p, args.plain_steps, 1)
+    out["plain_kernels"] = args.plain_steps * n / pdt
+    out["plain_align_ms"] = float(np.mean(pa))
+    out["plain_emit_ms"] = float(np.mean(pe))
+    log(f"c2 plain (resident): {out['plain_kernels'] / 1e6:.2f} M aln/s")
+    pres = r.results()
+    out["plain_equal"] = bool(np.array_equal(pres, res))
+
+    # plain end to end: per batch, host -> device -> host, synchronous
+    words_pin = ewfa.PinnedArray(w.words.size, np.uint64)
+    words_pin.array[:] = w.words
+    res_pin = ewfa.PinnedArray(args.batch, ewfa.RESULT_DTYPE)
+    cig_pin = ewfa.PinnedArray(args.batch * 1004, np.int8)
+    batches = []
+    for a in range(0, n, args.batch):
+        b = min(n, a + args.batch)
+        wv, sub, cb, _, _ = ewfa.slice_pairs(words_pin.array, w.pairs, a, b)
+        pp = ewfa.PinnedArray(len(sub), ewfa.PAIR_DTYPE)
+        pp.array[:] = sub
+        batches.append((wv, pp, b - a, cb))
+
+    def e2e(ctx_):
+        for wv, pp, nb, cb in batches:
+            ctx_.align_host(wv, pp.array, nb, cb, results=res_pin.array, cigars=cig_pin.array)
+
+    ctx_pe = ewfa.Context(mode=ewfa.MODE_PLAIN, batch_size=args.batch)
+    e2e(ctx_pe)
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        e2e(ctx_pe)
+    out["plain_e2e"] = args.plain_steps * n / (time.perf_counter() - t0)
+    log(f"c2 plain end to end: {out['plain_e2e'] / 1e6:.2f} M aln/s")
+    ctx_pe.close()
+
+    # persistent end to end: the whole job host -> device -> host in one call
+    res_all = ewfa.PinnedArray(n, ewfa.RESULT_DTYPE)
+    cig_all = ewfa.PinnedArray(w.cigar_bytes, np.int8)
+    pairs_pin = ewfa.PinnedArray(n, ewfa.PAIR_DTYPE)
+    pairs_pin.array[:] = w.pairs[:n]
+    ctx_e = ewfa.Context(mode=ewfa.MODE_PERSISTENT, policy_path=POLICY_C2)
+    run_e = lambda c: c.align_host(words_pin.array, pairs_pin.array, n, w.cigar_bytes,
+                                   results=res_all.array, cigars=cig_all.array)
+    run_e(ctx_e)
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        run_e(ctx_e)
+    out["persistent_e2e"] = args.plain_steps * n / (time.perf_counter() - t0)
+    log(f"c2 persistent end to end: {out['persistent_e2e'] / 1e6:.2f} M aln/s")
+    out["e2e_equal"] = bool(np.array_equal(res_all.array, res))
+    for c in (ctx, ctx_p, ctx_e):
+        c.close()
+    return out
+
+
+def leg_c3(args, threads):
+    import numpy as np
+    import ewfa
+    n = args.c3_pairs
+    t0 = time.time()
+    w = ewfa.Workload(ewfa.synth_spec(**ewfa.CONFIGS["c3"]), n, threads=threads)
+    log(f"c3: generated {n} pairs in {time.time() - t0:.1f}s, max_len {w.max_len}")
+    r = Resident(w)
+    ctx = ewfa.Context(mode=ewfa.MODE_PERSISTENT, flags=ewfa.FLAG_TIMING, policy_path=POLICY_C3)
+    dt, a_ms, e_ms = timed(r.run, ctx, args.c3_steps, 1)
+    ctx.close()
+    res = r.results()
+    out = {"value": args.c3_steps * n / dt, "align_ms": float(np.mean(a_ms)), "emit_ms": float(np.mean(e_ms)),
+           "ms_per_launch": dt / args.c3_steps * 1e3, "res": res, "cig": r.cigars(), "w": w}
+    out["cells"] = cells(res)
+    out["algo_bytes"] = algo_bytes(w, res)
+    log(f"c3: {out['value'] / 1e6:.2f} M aln/s, align {out['align_ms']:.2f} ms, emit {out['emit_ms']:.2f} ms")
+    return out
+
+
+def leg_c4(args, threads):
+    import numpy as np
+    import ewfa
+    n = args.c4_pairs
+    t0 = time.time()
+    w = ewfa.Workload(ewfa.synth_spec(**ewfa.CONFIGS["c4"]), n, threads=threads)
+    log(f"c4: generated {n} pairs in {time.time() - t0:.1f}s, max_len {w.max_len}")
+    r = Resident(w)
+    ctx = ewfa.Context(mode=ewfa.MODE_PERSISTENT, flags=ewfa.FLAG_TIMING)
+    dt, a_ms, e_ms = timed(r.run, ctx, 1, 1)
+    footprint = r.nbytes() + ctx.memory_bytes()
+    ctx.close()
+    res = r.results()
+    out = {"value": n / dt, "align_ms": float(np.mean(a_ms)), "emit_ms": float(np.mean(e_ms)),
+           "seconds": dt, "device_bytes": footprint, "res": res, "cig": r.cigars(), "w": w}
+    out["cells"] = cells(res)
+    log(f"c4: {out['value']:.0f} aln/s ({dt:.3f}s), device footprint {footprint / 2**30:.2f} GiB")
+    return out
+
+
+def leg_served(args, threads, batched_rate):
+    """Poisson arrivals of jobs into the job server.  A pool of distinct jobs
+    is submitted in rounds (each round a fresh random order); a job is not
+    resubmitted before its previous submission has completed."""
+    import numpy as np
+    import ewfa
+    rng = np.random.default_rng(5)
+    sizes = rng.integers(1, 4097, args.serve_pool)
+    total = int(sizes.sum())
+    spec = ewfa.synth_spec(kind=1, seed=5, len_lo=100, len_hi=10_000, err_lo=0.01, err_hi=0.10)
+    t0 = time.time()
+    w = ewfa.Workload(spec, total, threads=threads)
+    log(f"served: pool of {args.serve_pool} jobs, {total} pairs, generated in {time.time() - t0:.1f}s")
+    words_pin = ewfa.PinnedArray(w.words.size, np.uint64)
+    words_pin.array[:] = w.words
+    res_pin = ewfa.PinnedArray(total, ewfa.RESULT_DTYPE)
+    cig_pin = ewfa.PinnedArray(w.cigar_bytes, np.int8)
+    jobs = []
+    a = 0
+    for s in sizes:
+        b = a + int(s)
+        wv, sub, cb, _, c0 = ewfa.slice_pairs(words_pin.array, w.pairs, a, b)
+        jobs.append((wv, sub, b - a, cb, res_pin.array[a:b], cig_pin.array[c0:c0 + cb]))
+        a = b
+    mean_pairs = total / len(jobs)
+    lam = args.serve_load * batched_rate / mean_pairs
+    k = max(len(jobs), int(args.serve_seconds * lam))
+    order = np.concatenate([rng.permutation(len(jobs)) for _ in range((k + len(jobs) - 1) // len(jobs))])[:k]
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, k))
+    log(f"served: lambda {lam:.0f} jobs/s ({args.serve_load:.2f} x batched {batched_rate / 1e6:.2f} M aln/s), "
+        f"{k} submissions")
+
+    srv = ewfa.Server(slots=args.serve_slots, arena_bytes=args.serve_arena << 30)
+    inflight = [None] * len(jobs)
+    sub_t = np.zeros(k)
+    handles = []
+    throttled = 0
+    t_start = time.perf_counter()
+    for i in range(k):
+        target = t_start + arrivals[i]
+        while True:
+            now = time.perf_counter()
+            if now >= target:
+                break
+            time.sleep(min(target - now, 0.0005))
+        j = int(order[i])
+        if inflight[j] is not None and not inflight[j].done():
+            throttled += 1
+            inflight[j].wait()
+        wv, sub, nb, cb, ro, co = jobs[j]
+        sub_t[i] = time.perf_counter() - t_start
+        h = srv.submit(wv, sub, nb, cb, results=ro, cigars=co)
+        inflight[j] = h
+        handles.append((h, i, nb))
+    for h, _, _ in handles:
+        h.wait()
+    st = srv.stats()
+    lat = np.array([(sub_t[i] - arrivals[i]) + h.latency() for h, i, _ in handles])
+    finish = np.array([arrivals[i] + l for (h, i, _), l in zip(handles, lat)])
+    pairs_done = int(sum(nb for _, _, nb in handles))
+    span = float(finish.max() - arrivals[0])
+    srv.close()
+    out = {"lambda_jobs_per_s": round(lam, 1), "offered_frac_of_batched": args.serve_load,
+           "jobs": k, "pairs": pairs_done, "mean_job_pairs": round(mean_pairs, 1),
+           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+           "value": round(pairs_done / span, 1), "unit": "alignments/s",
+           "frac_of_batched": round(pairs_done / span / batched_rate, 3),
+           "batches": int(st["batches"]), "peak_batch_jobs": int(st["peak_batch_jobs"]),
+           "slots": args.serve_slots, "throttled_resubmits": throttled,
+           "arena_peak_bytes": int(st["arena_peak"]), "failed": int(st["failed"])}
+    # the served outputs of each job's last submission, checked below
+    out["w"], out["res"], out["cig"] = w, res_pin.array.copy(), cig_pin.array.copy()
+    log(f"served: {out['value'] / 1e6:.2f} M aln/s ({out['frac_of_batched']:.2f} of batched), "
+        f"p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, {out['batches']} launches")
+    return out
+
+
+def oracle_check(w, res, cig, count, threads):
+    """Distances and streams of `count` evenly spaced pairs against the oracle."""
+    import numpy as np
+    import ewfa
+    import oracle_py as orc
+    n = w.n
+    idx = np.linspace(0, n - 1, num=min(count, n)).astype(np.int64)
+    ok, od, osl, ost = orc.batch_workload(w, idx=idx, threads=threads)
+    got = ewfa.streams_from(res[idx], w.pairs[idx], cig)
+    bad = 0
+    for j, i in enumerate(idx):
+        if int(res[i]["distance"]) != int(od[j]) or not np.array_equal(got[j], ost[j]):
+            bad += 1
+    return len(idx), bad
+
+
+def validity_check(w, res, cig, count):
+    """Long pairs: every checked stream is a valid alignment of its pair
+    (the reference's CPU check) with as many edits as the distance."""
+    import numpy as np
+    import ewfa
+    idx = np.linspace(0, w.n - 1, num=min(count, w.n)).astype(np.int64)
+    streams = ewfa.streams_from(res[idx], w.pairs[idx], cig)
+    bad = 0
+    for j, i in enumerate(idx):
+        s = streams[j]
+        edits = int(np.isin(s, [ord("X"), ord("I"), ord("D")]).sum())
+        if ewfa.check_cigar(w.pattern(i), w.text(i), s) != 1 or edits != int(res[i]["distance"]):
+            bad += 1
+    return len(idx), bad
+
+
+def cpu_baseline(w, args, threads):
+    """The oracle on the host cores over a bounded prefix of the workload."""
+    import numpy as np
+    import oracle_py as orc
+    done = 0
+    t0 = time.perf_counter()
+    chunk = 100_000
+    while done < min(args.cpu_sample, w.n) and time.perf_counter() - t0 < args.cpu_seconds:
+        idx = np.arange(done, min(done + chunk, args.cpu_sample, w.n))
+        orc.batch_workload(w, idx=idx, threads=threads)
+        done += len(idx)
+    cdt = time.perf_counter() - t0
+    return done, cdt
+
+
+def cpu_c1(threads):
+    """c1 (10,000 x 150 bp, 2%) on the oracle at 1 thread and at `threads`."""
+    import ewfa
+    import oracle_py as orc
+    w = ewfa.Workload(ewfa.synth_spec(**ewfa.CONFIGS["c1"]), ewfa.CONFIGS["c1"]["n"])
+    out = {}
+    for t in (1, threads):
+        t0 = time.perf_counter()
+        orc.batch_workload(w, threads=t)
+        out[f"threads_{t}"] = round(w.n / (time.perf_counter() - t0), 1)
+    return out
 
 
 def main():
@@ -41,19 +373,28 @@ def main():
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--batch", type=int, default=100_000)
+    ap.add_argument("--legs", default="c2,c3,c4,served", help="comma list of c2,c3,c4,served")
+    ap.add_argument("--c3-pairs", type=int, default=500_000)
+    ap.add_argument("--c3-steps", type=int, default=3)
+    ap.add_argument("--c4-pairs", type=int, default=20_000)
+    ap.add_argument("--serve-load", type=float, default=0.85, help="offered load / batched c3 rate")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-pool", type=int, default=256)
+    ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-arena", type=int, default=16, help="GiB")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=3000, help="pairs checked against the oracle")
-    ap.add_argument("--profile-run", action="store_true",
-                    help="one eager persistent pass only (for rocprofv3)")
+    ap.add_argument("--profile-run", choices=["c2", "c3", "c4"],
+                    help="two eager persistent passes over one workload only (for rocprofv3)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="one 1 GiB device copy only (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
-              f"(pairs are independent; see DESIGN.md for how the work shards)", file=sys.stderr)
+              f"(pairs are independent and would shard by index range across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -61,6 +402,7 @@ def main():
     import ewfa
 
     assert ewfa.device_count() > 0, "bench needs a GPU"
+    threads = min(16, os.cpu_count() or 8)
 
     if args.profile_copy:
         x = ewfa.DeviceBuffer.from_array(np.ones(1 << 27, dtype=np.int64))
@@ -71,117 +413,48 @@ def main():
         log("copied 1 GiB")
         return
 
-    threads = min(16, os.cpu_count() or 8)
-    spec_d = dict(ewfa.CONFIGS["c2"])
-    spec = ewfa.synth_spec(**spec_d)
-    n = args.pairs
-    t0 = time.time()
-    w = ewfa.Workload(spec, n, cigar_cap=1004, threads=threads)
-    log(f"generated {n} pairs in {time.time() - t0:.1f}s, max_len {w.max_len}")
-
-    dw = ewfa.DeviceBuffer.from_array(w.words)
-    dp = ewfa.DeviceBuffer.from_array(w.pairs[:n])
-    dr = ewfa.DeviceBuffer(n * 16)
-    dc = ewfa.DeviceBuffer(w.cigar_bytes)
-    ewfa.synchronize()
-
-    def run(ctx):
-        ctx.align_device(dw.ptr, dp.ptr, n, w.max_len, dr.ptr, dc.ptr, None)
-
     if args.profile_run:
-        with ewfa.Context(mode=ewfa.MODE_PERSISTENT, flags=ewfa.FLAG_NO_GRAPH) as ctx:
-            run(ctx)
-            ewfa.synchronize()
-            run(ctx)
-            ewfa.synchronize()
-        log("profile run done")
+        name = args.profile_run
+        n = {"c2": args.pairs, "c3": args.c3_pairs, "c4": min(args.c4_pairs, 4000)}[name]
+        cap = 1004 if name == "c2" else 0
+        w = ewfa.Workload(ewfa.synth_spec(**ewfa.CONFIGS[name]), n, cigar_cap=cap, threads=threads)
+        r = Resident(w)
+        pol = {"c2": POLICY_C2, "c3": POLICY_C3, "c4": None}[name]
+        with ewfa.Context(mode=ewfa.MODE_PERSISTENT, flags=ewfa.FLAG_NO_GRAPH, policy_path=pol) as ctx:
+            for _ in range(2):
+                r.run(ctx)
+                ewfa.synchronize()
+        log(f"profile run {name} ({n} pairs) done")
         return
 
-    # persistent path (the value)
-    ctx = ewfa.Context(mode=ewfa.MODE_PERSISTENT)
-    for i in range(args.warmup):
-        run(ctx)
-        log(f"warmup {i + 1}/{args.warmup}")
-    ewfa.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run(ctx)
-    ewfa.synchronize()
-    dt = time.perf_counter() - t0
-    value = args.steps * n / dt
-    log(f"persistent: {args.steps} steps in {dt:.3f}s -> {value / 1e6:.2f} M alignments/s")
+    legs = set(args.legs.split(","))
+    c2 = leg_c2(args, threads)
+    c3 = leg_c3(args, threads) if "c3" in legs else None
+    c4 = leg_c4(args, threads) if "c4" in legs else None
+    served = leg_served(args, threads, c3["value"]) if ("served" in legs and c3) else None
 
-    # per-kernel durations from HIP events on the launch stream
-    ctx_t = ewfa.Context(mode=ewfa.MODE_PERSISTENT, flags=ewfa.FLAG_TIMING)
-    a_ms, e_ms = [], []
-    for _ in range(3):
-        run(ctx_t)
-        a, e = ctx_t.last_timing()
-        a_ms.append(a)
-        e_ms.append(e)
-    ctx_t.close()
-    align_ms = float(np.mean(a_ms))
-    emit_ms = float(np.mean(e_ms))
-    log(f"align kernel {align_ms:.3f} ms, emit kernel {emit_ms:.3f} ms (events)")
-
-    # plain path: the reference's per-batch launch model
-    ctx_plain = ewfa.Context(mode=ewfa.MODE_PLAIN, batch_size=args.batch)
-    run(ctx_plain)
-    ewfa.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.plain_steps):
-        run(ctx_plain)
-    ewfa.synchronize()
-    pdt = time.perf_counter() - t0
-    plain_value = args.plain_steps * n / pdt
-    log(f"plain: {plain_value / 1e6:.2f} M alignments/s")
-    ctx_plain.close()
-
-    # the persistent result again (the plain run overwrote the buffers)
-    run(ctx)
-    ewfa.synchronize()
-    res = dr.download(ewfa.RESULT_DTYPE, n)
-    cig = dc.download(np.int8)
-    ctx.close()
-    ok_status = int((res["status"] == ewfa.PAIR_OK).sum())
-    dist = res["distance"].astype(np.int64)
-    clen = res["cigar_len"].astype(np.int64)
-    plen = w.plen.astype(np.int64)
-    tlen = w.tlen.astype(np.int64)
-    algo_bytes = int(((plen + 3) // 4 + (tlen + 3) // 4 + 8 + clen).sum())
-    cells = int(((dist + 1) ** 2).sum())
-
-    # parity sample against the oracle
-    checked = 0
-    mismatches = 0
+    # parity samples against the oracle (and validity on the long reads)
+    checks = {}
     if args.check > 0:
-        import oracle_py as orc
-        idx = np.linspace(0, n - 1, num=min(args.check, n)).astype(np.int64)
-        ok, od, osl, ost = orc.batch_workload(w, idx=idx, threads=threads)
-        got = ewfa.streams_from(res[idx], w.pairs[idx], cig)
-        for j, i in enumerate(idx):
-            if int(res[i]["distance"]) != int(od[j]) or not np.array_equal(got[j], ost[j]):
-                mismatches += 1
-        checked = len(idx)
-        log(f"oracle check: {checked} pairs, {mismatches} mismatches")
+        checks["c2"] = oracle_check(c2["w"], c2["res"], c2["cig"], args.check, threads)
+        if c3:
+            checks["c3"] = oracle_check(c3["w"], c3["res"], c3["cig"], max(args.check // 10, 1), threads)
+        if served:
+            checks["served"] = oracle_check(served["w"], served["res"], served["cig"], max(args.check // 10, 1),
+                                            threads)
+        if c4:
+            checks["c4_valid"] = validity_check(c4["w"], c4["res"], c4["cig"], 200)
+        log("checks: " + ", ".join(f"{k} {v[0]} pairs {v[1]} bad" for k, v in checks.items()))
 
-    # CPU baseline: the oracle on a bounded sample of the same workload
     cpu = None
+    c1_cpu = None
     if not args.no_cpu:
-        import oracle_py as orc
-        cores = threads
-        done = 0
-        t0 = time.perf_counter()
-        chunk = 100_000
-        while done < args.cpu_sample and time.perf_counter() - t0 < args.cpu_seconds:
-            idx = np.arange(done, min(done + chunk, args.cpu_sample, n))
-            orc.batch_workload(w, idx=idx, threads=cores)
-            done += len(idx)
-        cdt = time.perf_counter() - t0
-        cpu = {"value": round(done / cdt, 1), "unit": "alignments/s", "cores": cores, "kind": "port",
-               "sample": f"first {done} pairs of the same workload (1000 bp, 5% edits), oracle align + "
-                         f"stream + text + validity check on {cores} host threads, {cdt:.1f}s"}
-        log(f"cpu baseline: {cpu['value']:.0f} alignments/s on {cores} threads")
+        done, cdt = cpu_baseline(c2["w"], args, threads)
+        cpu = {"value": round(done / cdt, 1), "unit": "alignments/s", "cores": threads, "kind": "port",
+               "sample": f"first {done} pairs of the same 1000 bp / 5% workload: oracle align + CIGAR stream "
+                         f"+ text + the reference's validity check, {threads} host threads, {cdt:.1f}s"}
+        c1_cpu = cpu_c1(threads)
+        log(f"cpu baseline: {cpu['value']:.0f} aln/s on {threads} threads; c1 {c1_cpu}")
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_align_c2.json")
@@ -189,36 +462,54 @@ def main():
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
-    achieved = algo_bytes / (align_ms / 1e3) / 1e9
+    n = args.pairs
+    achieved = c2["algo_bytes"] / (c2["align_ms"] / 1e3) / 1e9
     out = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": round(c2["value"], 1),
         "unit": "alignments/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "ms_per_step": round(c2["dt"] / args.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "c2: 1,000,000 pairs x 1,000 bp, 5% edits (sub/ins/del equal), seed 2, "
-                               "2-bit packed u64 input resident in HBM, CIGAR stride 1004 B",
+        "config": {"workload": "1,000,000 pairs x 1,000 bp, uniform ACGT, 5% edits (sub/ins/del equal), seed 2; "
+                               "2-bit packed u64 input resident in HBM, CIGAR stride 1004 B; persistent kernel, "
+                               "hipGraph replay, autotuned policy",
                    "pairs": n, "seq_len": 1000, "error": 0.05, "plain_batch": args.batch,
-                   "parallelism": "single GPU, persistent kernel"},
-        "plain": {"value": round(plain_value, 1), "unit": "alignments/s",
-                  "ms_per_step": round(pdt / args.plain_steps * 1e3, 3)},
-        "ratio_vs_plain": round(value / plain_value, 3),
-        "kernels_ms": {"align": round(align_ms, 4), "emit": round(emit_ms, 4)},
-        "cells_per_s": round(cells / (align_ms / 1e3), 1),
-        "pairs_ok": ok_status,
-        "oracle_checked": checked,
-        "oracle_mismatches": mismatches,
+                   "parallelism": "single GPU"},
+        "kernels_ms": {"align": round(c2["align_ms"], 4), "emit": round(c2["emit_ms"], 4)},
+        "cells_per_s": round(c2["cells"] / (c2["align_ms"] / 1e3), 1),
+        "plain": {"kernels": round(c2["plain_kernels"], 1), "end_to_end": round(c2["plain_e2e"], 1),
+                  "unit": "alignments/s", "align_ms": round(c2["plain_align_ms"], 3),
+                  "emit_ms": round(c2["plain_emit_ms"], 3), "identical_outputs": c2["plain_equal"]},
+        "persistent_end_to_end": round(c2["persistent_e2e"], 1),
+        "ratio_vs_plain": {"resident": round(c2["value"] / c2["plain_kernels"], 3),
+                           "end_to_end": round(c2["persistent_e2e"] / c2["plain_e2e"], 3),
+                           "value_vs_plain_end_to_end": round(c2["value"] / c2["plain_e2e"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic},
         "cpu_baseline": cpu,
     }
+    if c3:
+        out["c3"] = {"value": round(c3["value"], 1), "unit": "alignments/s", "pairs": c3["w"].n,
+                     "ms_per_launch": round(c3["ms_per_launch"], 3), "align_ms": round(c3["align_ms"], 3),
+                     "emit_ms": round(c3["emit_ms"], 3),
+                     "cells_per_s": round(c3["cells"] / (c3["align_ms"] / 1e3), 1)}
+    if c4:
+        out["c4"] = {"value": round(c4["value"], 1), "unit": "alignments/s", "pairs": c4["w"].n,
+                     "seconds": round(c4["seconds"], 4), "align_ms": round(c4["align_ms"], 3),
+                     "emit_ms": round(c4["emit_ms"], 3), "device_bytes": int(c4["device_bytes"]),
+                     "cells_per_s": round(c4["cells"] / c4["seconds"], 1)}
+    if served:
+        out["served"] = {k: v for k, v in served.items() if k not in ("w", "res", "cig")}
+    if c1_cpu:
+        out["cpu_c1"] = dict(c1_cpu, unit="alignments/s", sample="c1: 10,000 x 150 bp, 2% edits, whole")
+    out["oracle_checks"] = {k: {"pairs": v[0], "mismatches": v[1]} for k, v in checks.items()}
     print(json.dumps(out), flush=True)
 
 

@@ -95,18 +95,23 @@ struct ewfa_ctx {
     float align_ms = 0.f, emit_ms = 0.f;
     bool timing = false;
     // graph cache for the persistent path
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr, graph2 = nullptr;  // graph2: emit part when timed
     struct Key {
-        const void *words, *pairs, *results, *cigars;
+        const void *words, *pairs, *results, *cigars, *ops;
         size_t n;
         uint32_t max_len;
+        bool timing;
         bool operator==(const Key& o) const {
             return words == o.words && pairs == o.pairs && results == o.results &&
-                   cigars == o.cigars && n == o.n && max_len == o.max_len;
+                   cigars == o.cigars && ops == o.ops && n == o.n && max_len == o.max_len &&
+                   timing == o.timing;
         }
     } graph_key{};
     bool use_graph = true;
     int policy_gen = 0, graph_policy_gen = -1;
+    // op scratch supplied by the caller for one call (the job server's arena)
+    uint64_t* ops_ext = nullptr;
+    size_t ops_ext_bytes = 0;
     // host pipeline buffers
     DevBuf h_words, h_pairs, h_results, h_cigars;
     size_t resident_words = 0;
@@ -259,9 +264,17 @@ int next_pass(ewfa_ctx* c, uint32_t max_len, int prev_dcap, uint32_t items, Pass
     return finish_pass(c, p, items);
 }
 
+uint64_t* ops_base(ewfa_ctx* c) { return c->ops_ext ? c->ops_ext : c->ops.as<uint64_t>(); }
+
 int reserve_scratch(ewfa_ctx* c, size_t n, uint32_t max_len) {
     const uint32_t ops_stride = ewfa::ops_words(max_len + 1);
-    CHK(c->ops.reserve((size_t)n * ops_stride * 8));
+    const size_t ops_bytes = (size_t)n * ops_stride * 8;
+    if (c->ops_ext) {
+        if (c->ops_ext_bytes < ops_bytes)
+            return fail(EWFA_ERR_RANGE, "op scratch of %zu bytes, %zu needed", c->ops_ext_bytes, ops_bytes);
+    } else {
+        CHK(c->ops.reserve(ops_bytes));
+    }
     CHK(c->list_a.reserve((size_t)std::max<size_t>(n, 1) * 4));
     CHK(c->list_b.reserve((size_t)std::max<size_t>(n, 1) * 4));
     return EWFA_OK;
@@ -293,7 +306,7 @@ ewfa::AlignArgs base_args(ewfa_ctx* c, const uint64_t* words, const ewfa_pair_t*
     a.words = words;
     a.pairs = pairs;
     a.results = results;
-    a.ops = c->ops.as<uint64_t>();
+    a.ops = ops_base(c);
     a.ops_stride = ewfa::ops_words(max_len + 1);
     a.max_distance = c->opts.max_distance;
     a.max_len = max_len;
@@ -332,7 +345,7 @@ int enqueue_main(ewfa_ctx* c, const Pass& p, const uint64_t* words, const ewfa_p
     // batch-local view: shift the pair/result bases, the ops scratch is per pair
     a.pairs = pairs + first;
     a.results = results + first;
-    a.ops = c->ops.as<uint64_t>() + first * (size_t)a.ops_stride;
+    a.ops = ops_base(c) + first * (size_t)a.ops_stride;
     a.list = nullptr;
     a.count = count;
     a.count_ptr = nullptr;
@@ -365,34 +378,42 @@ int enqueue_main(ewfa_ctx* c, const Pass& p, const uint64_t* words, const ewfa_p
 
 // Enqueues a persistent plan: bucketing, one align pass per bucket (longest
 // first), the emit per bucket, and the deferred count's copy to the host.
+// phase: PLAN_ALL, or PLAN_ALIGN (bucketing and align passes) then PLAN_EMIT
+// (emits and the count's copy) as two graphs timed between their launches.
+enum { PLAN_ALL = 3, PLAN_ALIGN = 1, PLAN_EMIT = 2 };
+
 int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_pair_t* pairs, uint32_t n,
-                 uint32_t max_len, ewfa_result_t* results, int8_t* cigars, hipStream_t s, bool timing) {
+                 uint32_t max_len, ewfa_result_t* results, int8_t* cigars, hipStream_t s, bool timing,
+                 int phase = PLAN_ALL) {
     uint32_t* ctrl = c->ctrl.as<uint32_t>();
     uint32_t* lists = c->buckets.as<uint32_t>();
     const bool bucketed = pl.nb > 1;
-    HIPCHK(ewfa::launch_fill_u32(ctrl, 0u, CTRL_BCOUNT + ewfa::MAX_BUCKETS, s));
-    if (bucketed) HIPCHK(ewfa::launch_bucketize(pairs, n, pl.table, lists, ctrl + CTRL_BCOUNT, s));
-    if (timing) HIPCHK(hipEventRecord(c->ev_a0, s));
-    for (int b = pl.nb - 1; b >= 0; --b) {
-        const Pass& p = pl.pass[b];
-        ewfa::AlignArgs a = base_args(c, words, pairs, results, max_len);
-        fill_pass_args(a, p, c);
-        a.list = bucketed ? lists + (size_t)b * n : nullptr;
-        a.count = n;
-        a.count_ptr = bucketed ? ctrl + CTRL_BCOUNT + b : nullptr;
-        a.queue = ctrl + CTRL_BQUEUE + b;
-        a.defer_list = c->list_a.as<uint32_t>();
-        a.defer_count = ctrl + CTRL_DEFER_A;
-        a.max_len = pl.blen[b];
-        CHK(launch_pass(a, p, s));
+    if (phase & PLAN_ALIGN) {
+        HIPCHK(ewfa::launch_fill_u32(ctrl, 0u, CTRL_BCOUNT + ewfa::MAX_BUCKETS, s));
+        if (bucketed) HIPCHK(ewfa::launch_bucketize(pairs, n, pl.table, lists, ctrl + CTRL_BCOUNT, s));
+        if (timing) HIPCHK(hipEventRecord(c->ev_a0, s));
+        for (int b = pl.nb - 1; b >= 0; --b) {
+            const Pass& p = pl.pass[b];
+            ewfa::AlignArgs a = base_args(c, words, pairs, results, max_len);
+            fill_pass_args(a, p, c);
+            a.list = bucketed ? lists + (size_t)b * n : nullptr;
+            a.count = n;
+            a.count_ptr = bucketed ? ctrl + CTRL_BCOUNT + b : nullptr;
+            a.queue = ctrl + CTRL_BQUEUE + b;
+            a.defer_list = c->list_a.as<uint32_t>();
+            a.defer_count = ctrl + CTRL_DEFER_A;
+            a.max_len = pl.blen[b];
+            CHK(launch_pass(a, p, s));
+        }
+        if (timing) HIPCHK(hipEventRecord(c->ev_a1, s));
     }
-    if (timing) HIPCHK(hipEventRecord(c->ev_a1, s));
+    if (!(phase & PLAN_EMIT)) return EWFA_OK;
     for (int b = pl.nb - 1; b >= 0; --b) {
         ewfa::EmitArgs e{};
         e.words = words;
         e.pairs = pairs;
         e.results = results;
-        e.ops = c->ops.as<uint64_t>();
+        e.ops = ops_base(c);
         e.ops_stride = ewfa::ops_words(max_len + 1);
         e.cigars = cigars;
         e.list = bucketed ? lists + (size_t)b * n : nullptr;
@@ -437,7 +458,7 @@ int resolve_deferred(ewfa_ctx* c, int prev_dcap, const uint64_t* words, const ew
         fill_pass_args(a, p, c);
         a.pairs = pairs + first;
         a.results = results + first;
-        a.ops = c->ops.as<uint64_t>() + first * (size_t)a.ops_stride;
+        a.ops = ops_base(c) + first * (size_t)a.ops_stride;
         a.list = in_list;
         a.count = pending;
         a.count_ptr = nullptr;
@@ -470,6 +491,21 @@ int resolve_deferred(ewfa_ctx* c, int prev_dcap, const uint64_t* words, const ew
 }  // namespace
 
 namespace ewfa {
+size_t op_scratch_bytes(size_t n, uint32_t max_len) { return n * (size_t)ops_words(max_len + 1) * 8; }
+
+int align_device_with_scratch(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t* d_pairs, size_t n,
+                              uint32_t max_len, ewfa_result_t* d_results, int8_t* d_cigars, void* stream,
+                              uint64_t* d_ops, size_t ops_bytes) {
+    if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    if (!d_ops) return fail(EWFA_ERR_ARG, "null op scratch");
+    c->ops_ext = d_ops;
+    c->ops_ext_bytes = ops_bytes;
+    const int rc = ewfa_align_device(c, d_words, d_pairs, n, max_len, d_results, d_cigars, stream);
+    c->ops_ext = nullptr;
+    c->ops_ext_bytes = 0;
+    return rc;
+}
+
 int set_context_policy(ewfa_ctx_t* c, const Policy& p) {
     if (!c || p.buckets.empty()) return fail(EWFA_ERR_ARG, "empty policy");
     c->policy = p;
@@ -568,6 +604,7 @@ int ewfa_destroy(ewfa_ctx_t* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->graph2) (void)hipGraphExecDestroy(c->graph2);
     for (hipEvent_t e : {c->ev_in, c->ev_a0, c->ev_a1, c->ev_e1})
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -644,6 +681,16 @@ int ewfa_pack_ascii(const char* d_ascii, size_t ascii_stride, const uint32_t* d_
     return EWFA_OK;
 }
 
+int ewfa_context_memory(ewfa_ctx_t* c, size_t* bytes) {
+    if (!c || !bytes) return fail(EWFA_ERR_ARG, "null argument");
+    size_t b = 0;
+    for (const DevBuf* d : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->h_words,
+                            &c->h_pairs, &c->h_results, &c->h_cigars})
+        b += d->bytes;
+    *bytes = b;
+    return EWFA_OK;
+}
+
 int ewfa_max_distance(ewfa_ctx_t* c, uint32_t max_len) {
     if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
     int most = max_dcap_for(seq_words_for(max_len), 0, lds_budget(c));
@@ -684,38 +731,55 @@ int ewfa_align_device(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t*
         Plan pl;
         CHK(plan_persistent(c, max_len, (uint32_t)n, pl));
         CHK(reserve_plan(c, pl, n));
-        ewfa_ctx::Key key{d_words, d_pairs, d_results, d_cigars, n, max_len};
         const bool timing = c->timing;
-        if (c->use_graph && !timing) {
+        ewfa_ctx::Key key{d_words, d_pairs, d_results, d_cigars, ops_base(c), n, max_len, timing};
+        if (c->use_graph) {
+            // with timing: two graphs (align, emit) with events between them
+            const int nparts = timing ? 2 : 1;
             if (!(c->graph && c->graph_key == key && c->graph_policy_gen == c->policy_gen)) {
-                if (c->graph) {
-                    (void)hipGraphExecDestroy(c->graph);
-                    c->graph = nullptr;
-                }
-                hipGraph_t g = nullptr;
-                HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                int rc = enqueue_plan(c, pl, d_words, d_pairs, (uint32_t)n, max_len, d_results, d_cigars, s, false);
-                hipError_t ce = hipStreamEndCapture(s, &g);
-                if (rc != EWFA_OK) {
-                    if (g) (void)hipGraphDestroy(g);
-                    return rc;
-                }
-                if (ce != hipSuccess) return fail(EWFA_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
-                hipError_t ie = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                if (ie != hipSuccess) {
-                    c->graph = nullptr;
-                    return fail(EWFA_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ie));
+                for (hipGraphExec_t* ge : {&c->graph, &c->graph2})
+                    if (*ge) {
+                        (void)hipGraphExecDestroy(*ge);
+                        *ge = nullptr;
+                    }
+                for (int part = 0; part < nparts; ++part) {
+                    const int phase = timing ? (part == 0 ? PLAN_ALIGN : PLAN_EMIT) : PLAN_ALL;
+                    hipGraph_t g = nullptr;
+                    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    int rc = enqueue_plan(c, pl, d_words, d_pairs, (uint32_t)n, max_len, d_results, d_cigars, s,
+                                          false, phase);
+                    hipError_t ce = hipStreamEndCapture(s, &g);
+                    if (rc != EWFA_OK) {
+                        if (g) (void)hipGraphDestroy(g);
+                        return rc;
+                    }
+                    if (ce != hipSuccess) return fail(EWFA_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+                    hipGraphExec_t* ge = part == 0 ? &c->graph : &c->graph2;
+                    hipError_t ie = hipGraphInstantiate(ge, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    if (ie != hipSuccess) {
+                        *ge = nullptr;
+                        return fail(EWFA_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ie));
+                    }
                 }
                 c->graph_key = key;
                 c->graph_policy_gen = c->policy_gen;
             }
-            HIPCHK(hipEventRecord(c->ev_a0, s));
-            HIPCHK(hipGraphLaunch(c->graph, s));
-            HIPCHK(hipEventRecord(c->ev_e1, s));
+            if (timing) {
+                HIPCHK(hipEventRecord(c->ev_a0, s));
+                HIPCHK(hipGraphLaunch(c->graph, s));
+                HIPCHK(hipEventRecord(c->ev_a1, s));
+                HIPCHK(hipGraphLaunch(c->graph2, s));
+                HIPCHK(hipEventRecord(c->ev_e1, s));
+            } else {
+                HIPCHK(hipGraphLaunch(c->graph, s));
+            }
             HIPCHK(hipStreamSynchronize(s));
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, c->ev_a0, c->ev_e1) == hipSuccess) c->align_ms = ms;
+            if (timing) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, c->ev_a0, c->ev_a1) == hipSuccess) c->align_ms = ms;
+                if (hipEventElapsedTime(&ms, c->ev_a1, c->ev_e1) == hipSuccess) c->emit_ms = ms;
+            }
         } else {
             CHK(enqueue_plan(c, pl, d_words, d_pairs, (uint32_t)n, max_len, d_results, d_cigars, s, timing));
             HIPCHK(hipStreamSynchronize(s));

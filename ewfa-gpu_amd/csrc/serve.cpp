@@ -1,6 +1,8 @@
-// serve.cpp -- the job server: continuous admission of alignment jobs into
-// device slots by arena footprint, concurrent execution on per-slot streams,
-// per-job latency.
+// serve.cpp -- the job server: continuous batching of alignment jobs.  Each
+// slot (a worker thread with its own context and stream) takes every queued
+// job that fits the device arena, merges them into one persistent launch and
+// returns each job's results; jobs that arrive meanwhile form the next batch
+// on the next free slot, so batches overlap on the device.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
@@ -16,6 +18,7 @@
 #include <vector>
 
 #include "ewfa.h"
+#include "policy.h"
 
 namespace {
 
@@ -94,10 +97,28 @@ struct Job {
     int8_t* cigars;
     size_t cigar_bytes;
     uint32_t max_len;
-    size_t footprint;
+    size_t data_bytes;  // resident block: packed words, then CIGAR streams
+    size_t data_off = SIZE_MAX;
     Clock::time_point submitted, finished;
     int rc = EWFA_OK;
     bool done = false;
+};
+
+size_t words_block(const Job* j) { return Arena::round(j->n_words * 8); }
+
+// Per-launch state of a batch of n pairs up to max_len bases: merged
+// descriptors, results and the op scratch of the wavefront traceback.
+size_t state_bytes(size_t n, uint32_t max_len) {
+    return Arena::round(n * sizeof(ewfa_pair_t)) + Arena::round(n * sizeof(ewfa_result_t)) +
+           Arena::round(ewfa::op_scratch_bytes(n, max_len));
+}
+
+// Jobs merged into one launch.
+struct Batch {
+    std::vector<Job*> jobs;
+    size_t n = 0;
+    uint32_t max_len = 0;
+    size_t state_off = SIZE_MAX, state = 0;
 };
 
 }  // namespace
@@ -113,37 +134,107 @@ struct ewfa_server {
     std::vector<std::thread> workers;
     uint64_t next_id = 1;
     bool stopping = false;
-    int running = 0, peak_running = 0;
-    uint64_t submitted = 0, completed = 0, failed = 0;
+    int running = 0, peak_running = 0, peak_batch_jobs = 0;
+    uint64_t submitted = 0, completed = 0, failed = 0, batches = 0;
 };
 
 namespace {
 
-size_t footprint_of(size_t n_words, size_t n, size_t cigar_bytes) {
-    return Arena::round(n_words * 8) + Arena::round(n * sizeof(ewfa_pair_t)) +
-           Arena::round(n * sizeof(ewfa_result_t)) + Arena::round(cigar_bytes);
+// Takes jobs from the head of the queue into b while their resident blocks
+// and the batch's growing launch state fit the arena together (first come,
+// first served: a head job that does not fit waits, and so does everything
+// behind it).  Called with the server lock held; false if nothing fit.
+bool form_batch(ewfa_server* s, Batch& b) {
+    while (!s->queue.empty()) {
+        Job* j = s->queue.front();
+        if (!b.jobs.empty() && b.n + j->n > s->opts.max_batch_pairs) break;
+        const uint32_t ml = std::max(b.max_len, j->max_len);
+        const size_t st = state_bytes(b.n + j->n, ml);
+        const size_t off = s->arena.alloc(j->data_bytes);
+        if (off == SIZE_MAX) break;
+        const size_t probe = s->arena.alloc(st);
+        if (probe == SIZE_MAX) {
+            s->arena.free(off, j->data_bytes);
+            break;
+        }
+        s->arena.free(probe, st);
+        j->data_off = off;
+        b.jobs.push_back(j);
+        b.n += j->n;
+        b.max_len = ml;
+        s->queue.pop_front();
+    }
+    if (b.jobs.empty()) return false;
+    b.state = state_bytes(b.n, b.max_len);
+    b.state_off = s->arena.alloc(b.state);  // the last probe of this size succeeded
+    return b.state_off != SIZE_MAX;
 }
 
-// Runs one job on the slot's context: upload into its arena block, align,
-// download.  Returns an EWFA status.
-int run_job(ewfa_server* s, ewfa_ctx_t* ctx, hipStream_t st, Job* j, size_t base) {
-    char* p = s->arena.at(base);
-    char* d_words = p;
-    char* d_pairs = d_words + Arena::round(j->n_words * 8);
-    char* d_results = d_pairs + Arena::round(j->n * sizeof(ewfa_pair_t));
-    char* d_cigars = d_results + Arena::round(j->n * sizeof(ewfa_result_t));
-    if (hipMemcpyAsync(d_words, j->words, j->n_words * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(d_pairs, j->pairs, j->n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, st) != hipSuccess)
+// Pinned staging for a batch's merged descriptors.
+struct Staging {
+    ewfa_pair_t* p = nullptr;
+    size_t cap = 0;
+    ~Staging() {
+        if (p) (void)hipHostFree(p);
+    }
+    bool reserve(size_t n) {
+        if (n <= cap) return true;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipHostMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(ewfa_pair_t), hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        cap = n;
+        return true;
+    }
+};
+
+// Runs a batch as one launch: uploads each job's packed words into its
+// block, rebases its descriptors onto the arena (sequence offsets in words,
+// CIGAR offsets in bytes, both from the arena base), aligns all pairs
+// together and copies each job's results and CIGAR streams back.
+int run_batch(ewfa_server* s, ewfa_ctx_t* ctx, hipStream_t st, Staging& stage, const Batch& b) {
+    char* base = s->arena.at(0);
+    if (!stage.reserve(b.n)) return EWFA_ERR_NOMEM;
+    size_t k = 0;
+    for (const Job* j : b.jobs) {
+        if (j->n_words &&
+            hipMemcpyAsync(base + j->data_off, j->words, j->n_words * 8, hipMemcpyHostToDevice, st) != hipSuccess)
+            return EWFA_ERR_HIP;
+        const uint64_t woff = j->data_off / 8;
+        const uint64_t coff = j->data_off + words_block(j);
+        for (size_t i = 0; i < j->n; ++i) {
+            ewfa_pair_t p = j->pairs[i];
+            p.pattern_off += woff;
+            p.text_off += woff;
+            p.cigar_off += coff;
+            stage.p[k++] = p;
+        }
+    }
+    if (b.n == 0) return EWFA_OK;
+    char* d_pairs = base + b.state_off;
+    char* d_results = d_pairs + Arena::round(b.n * sizeof(ewfa_pair_t));
+    char* d_ops = d_results + Arena::round(b.n * sizeof(ewfa_result_t));
+    if (hipMemcpyAsync(d_pairs, stage.p, b.n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, st) != hipSuccess)
         return EWFA_ERR_HIP;
-    int rc = ewfa_align_device(ctx, reinterpret_cast<const uint64_t*>(d_words),
-                               reinterpret_cast<const ewfa_pair_t*>(d_pairs), j->n, j->max_len,
-                               reinterpret_cast<ewfa_result_t*>(d_results), reinterpret_cast<int8_t*>(d_cigars), st);
+    const int rc = ewfa::align_device_with_scratch(
+        ctx, reinterpret_cast<const uint64_t*>(base), reinterpret_cast<const ewfa_pair_t*>(d_pairs), b.n, b.max_len,
+        reinterpret_cast<ewfa_result_t*>(d_results), reinterpret_cast<int8_t*>(base), st,
+        reinterpret_cast<uint64_t*>(d_ops), ewfa::op_scratch_bytes(b.n, b.max_len));
     if (rc != EWFA_OK) return rc;
-    if (hipMemcpyAsync(j->results, d_results, j->n * sizeof(ewfa_result_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(j->cigars, d_cigars, j->cigar_bytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return EWFA_ERR_HIP;
-    return EWFA_OK;
+    k = 0;
+    for (const Job* j : b.jobs) {
+        if (j->n &&
+            (hipMemcpyAsync(j->results, d_results + k * sizeof(ewfa_result_t), j->n * sizeof(ewfa_result_t),
+                            hipMemcpyDeviceToHost, st) != hipSuccess ||
+             hipMemcpyAsync(j->cigars, base + j->data_off + words_block(j), j->cigar_bytes, hipMemcpyDeviceToHost,
+                            st) != hipSuccess))
+            return EWFA_ERR_HIP;
+        k += j->n;
+    }
+    return hipStreamSynchronize(st) == hipSuccess ? EWFA_OK : EWFA_ERR_HIP;
 }
 
 void worker_main(ewfa_server* s) {
@@ -155,38 +246,40 @@ void worker_main(ewfa_server* s) {
     o.policy_path = s->policy.empty() ? nullptr : s->policy.c_str();
     ewfa_ctx_t* ctx = nullptr;
     const int crc = ewfa_create(&ctx, &o);
-    if (crc == EWFA_OK) ewfa_set_flags(ctx, EWFA_FLAG_NO_GRAPH);  // each job has its own buffers
+    if (crc == EWFA_OK) ewfa_set_flags(ctx, EWFA_FLAG_NO_GRAPH);  // every batch has its own shape
     hipStream_t st = nullptr;
     (void)hipSetDevice(s->opts.device);
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
+    Staging stage;
     for (;;) {
-        Job* j = nullptr;
-        size_t base = SIZE_MAX;
+        Batch b;
         {
             std::unique_lock<std::mutex> lk(s->m);
-            // first come, first served: the head job waits for arena room
             s->cv_work.wait(lk, [&] {
                 if (s->queue.empty()) return s->stopping;
-                base = s->arena.alloc(s->queue.front()->footprint);
-                return base != SIZE_MAX;
+                return form_batch(s, b);
             });
-            if (s->queue.empty()) break;
-            j = s->queue.front();
-            s->queue.pop_front();
-            ++s->running;
+            if (b.jobs.empty()) break;  // stopping, queue drained
+            s->running += (int)b.jobs.size();
             s->peak_running = std::max(s->peak_running, s->running);
+            s->peak_batch_jobs = std::max(s->peak_batch_jobs, (int)b.jobs.size());
+            ++s->batches;
         }
-        s->cv_work.notify_all();  // the next head may fit beside this one
-        int rc = (crc != EWFA_OK || !st) ? EWFA_ERR_HIP : run_job(s, ctx, st, j, base);
+        s->cv_work.notify_all();  // the next head may fit beside this batch
+        const int rc = (crc != EWFA_OK || !st) ? EWFA_ERR_HIP : run_batch(s, ctx, st, stage, b);
         {
             std::lock_guard<std::mutex> lk(s->m);
-            s->arena.free(base, j->footprint);
-            j->rc = rc;
-            j->finished = Clock::now();
-            j->done = true;
-            --s->running;
-            ++s->completed;
-            if (rc != EWFA_OK) ++s->failed;
+            const auto now = Clock::now();
+            for (Job* j : b.jobs) {
+                s->arena.free(j->data_off, j->data_bytes);
+                j->rc = rc;
+                j->finished = now;
+                j->done = true;
+                ++s->completed;
+                if (rc != EWFA_OK) ++s->failed;
+            }
+            s->arena.free(b.state_off, b.state);
+            s->running -= (int)b.jobs.size();
         }
         s->cv_work.notify_all();
         s->cv_done.notify_all();
@@ -208,6 +301,7 @@ int ewfa_server_create(ewfa_server_t** out, const ewfa_server_opts_t* opts) {
     if (s->opts.slots <= 0) s->opts.slots = 4;
     if (s->opts.slots > 64) s->opts.slots = 64;
     if (s->opts.arena_bytes == 0) s->opts.arena_bytes = (size_t)4 << 30;
+    if (s->opts.max_batch_pairs == 0) s->opts.max_batch_pairs = (size_t)1 << 18;
     if (s->opts.policy_path) s->policy = s->opts.policy_path;
     s->opts.policy_path = nullptr;
     const int rc = s->arena.init(s->opts.device, s->opts.arena_bytes);
@@ -256,8 +350,8 @@ int ewfa_server_submit(ewfa_server_t* s, const uint64_t* h_words, size_t n_words
     j->cigars = h_cigars;
     j->cigar_bytes = cigar_bytes;
     j->max_len = max_len;
-    j->footprint = footprint_of(n_words, n, cigar_bytes);
-    if (j->footprint > s->arena.capacity()) {
+    j->data_bytes = Arena::round(n_words * 8) + Arena::round(cigar_bytes);
+    if (j->data_bytes + state_bytes(n, max_len) > s->arena.capacity()) {
         delete j;
         return EWFA_ERR_RANGE;
     }
@@ -312,6 +406,8 @@ int ewfa_server_stats(ewfa_server_t* s, ewfa_server_stats_t* out) {
     out->arena_peak = s->arena.peak();
     out->slots = s->opts.slots;
     out->peak_running = s->peak_running;
+    out->batches = s->batches;
+    out->peak_batch_jobs = s->peak_batch_jobs;
     return EWFA_OK;
 }
 

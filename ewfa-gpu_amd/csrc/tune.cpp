@@ -39,13 +39,13 @@ std::vector<ewfa::BucketPolicy> candidates(uint32_t blen) {
     // one wave per pair, wavefront in registers (levels up to 32*chunks-1)
     for (int dcap : {31, 63, 95, 127}) {
         if (dcap > dmax && dcap != 31) continue;
-        for (int grab : {4, 16}) c.push_back({0xFFFFFFFFu, 1, dcap, 1, grab});
+        for (int grab : {1, 4, 16}) c.push_back({0xFFFFFFFFu, 1, dcap, 1, grab});
     }
     // one wave per pair, offsets in LDS
     if (blen >= 400) c.push_back({0xFFFFFFFFu, 1, std::min(255, dmax), 1, 4});
     // a block per pair, backtrace in HBM
     if (blen >= 400)
-        for (int waves : {4, 8, 16})
+        for (int waves : {2, 4, 8, 16})
             for (int dcap : {512, 2048, 1 << 20}) {
                 if (dcap > 4 * dmax && dcap != 512) continue;
                 c.push_back({0xFFFFFFFFu, waves, dcap, 0, 1});

@@ -52,13 +52,14 @@ class CliArgs(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("arena_bytes", ctypes.c_size_t),
-                ("max_distance", ctypes.c_int), ("policy_path", ctypes.c_char_p)]
+                ("max_distance", ctypes.c_int), ("policy_path", ctypes.c_char_p),
+                ("max_batch_pairs", ctypes.c_size_t)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
                 ("arena_bytes", ctypes.c_size_t), ("arena_peak", ctypes.c_size_t), ("slots", ctypes.c_int),
-                ("peak_running", ctypes.c_int)]
+                ("peak_running", ctypes.c_int), ("batches", ctypes.c_uint64), ("peak_batch_jobs", ctypes.c_int)]
 
 
 class PairsFile(ctypes.Structure):
@@ -78,6 +79,7 @@ EXPORTS = [
     "ewfa_device_alloc", "ewfa_device_free", "ewfa_memcpy", "ewfa_device_synchronize",
     "ewfa_server_create", "ewfa_server_destroy", "ewfa_server_submit", "ewfa_server_poll", "ewfa_server_wait",
     "ewfa_server_job_latency", "ewfa_server_stats", "ewfa_autotune", "ewfa_host_alloc", "ewfa_host_free",
+    "ewfa_context_memory",
 ]
 
 _lib = None
@@ -126,6 +128,7 @@ def lib():
     L.ewfa_memcpy.argtypes = [_vp, _vp, _sz, ctypes.c_int]
     L.ewfa_device_synchronize.argtypes = [ctypes.c_int]
     L.ewfa_host_alloc.argtypes = [_sz, ctypes.POINTER(_vp)]
+    L.ewfa_context_memory.argtypes = [_vp, ctypes.POINTER(_sz)]
     L.ewfa_host_free.argtypes = [_vp]
     L.ewfa_server_create.argtypes = [ctypes.POINTER(_vp), ctypes.POINTER(ServerOpts)]
     L.ewfa_server_destroy.argtypes = [_vp]
@@ -220,6 +223,24 @@ class Workload:
         return self.txt[i * self.stride: i * self.stride + int(self.tlen[i])].tobytes()
 
 
+def slice_pairs(words, pairs, a, b):
+    """Pairs [a, b) of a sequential layout as a self-contained job:
+    (words view, rebased descriptors, cigar_bytes, first word, first CIGAR byte).
+    The words view shares memory with `words`."""
+    sub = np.array(pairs[a:b], dtype=PAIR_DTYPE, copy=True)
+    if b <= a:
+        return words[:1], np.zeros(1, dtype=PAIR_DTYPE), 4, 0, 0
+    w0 = int(sub[0]["pattern_off"])
+    last = sub[-1]
+    w1 = int(last["text_off"]) + int(lib().ewfa_packed_words(int(last["tlen"])))
+    c0 = int(sub[0]["cigar_off"])
+    c1 = int(last["cigar_off"]) + int(last["cigar_cap"])
+    sub["pattern_off"] -= w0
+    sub["text_off"] -= w0
+    sub["cigar_off"] -= c0
+    return words[w0:w1], sub, c1 - c0, w0, c0
+
+
 def from_sequences(patterns, texts, cigar_cap=0):
     """Lays out explicit (pattern, text) byte strings: (words, pairs, cigar_bytes, max_len)."""
     L = lib()
@@ -312,6 +333,9 @@ class DeviceBuffer:
         assert arr.nbytes <= self.nbytes
         _check(lib().ewfa_memcpy(self.ptr, arr.ctypes.data, arr.nbytes, 0))
 
+    def fill_zero(self):
+        self.upload(np.zeros(self.nbytes, dtype=np.uint8))
+
     def download(self, dtype=np.uint8, count=None):
         dtype = np.dtype(dtype)
         if count is None:
@@ -397,18 +421,28 @@ class Context:
     def align_device(self, d_words, d_pairs, n, max_len, d_results, d_cigars, stream=None):
         _check(lib().ewfa_align_device(self.handle, d_words, d_pairs, n, max_len, d_results, d_cigars, stream))
 
-    def align_host(self, words, pairs, n, cigar_bytes):
-        """Host arrays in, (results, cigars) host arrays out."""
-        results = np.zeros(max(n, 1), dtype=RESULT_DTYPE)
-        cigars = np.zeros(max(cigar_bytes, 4), dtype=np.int8)
+    def align_host(self, words, pairs, n, cigar_bytes, results=None, cigars=None):
+        """Host arrays in, (results, cigars) host arrays out (fresh arrays, or
+        the caller's, e.g. views of pinned memory)."""
+        if results is None:
+            results = np.zeros(max(n, 1), dtype=RESULT_DTYPE)
+        if cigars is None:
+            cigars = np.zeros(max(cigar_bytes, 4), dtype=np.int8)
+        assert results.size >= n and cigars.nbytes >= cigar_bytes
         _check(lib().ewfa_align_host(self.handle, words.ctypes.data, words.size, pairs.ctypes.data, n,
-                                     results.ctypes.data, cigars.ctypes.data, cigars.size))
+                                     results.ctypes.data, cigars.ctypes.data, min(cigars.nbytes, max(cigar_bytes, 4))))
         return results[:n], cigars
 
     def last_timing(self):
         a, e = ctypes.c_float(), ctypes.c_float()
         _check(lib().ewfa_last_timing(self.handle, ctypes.byref(a), ctypes.byref(e)))
         return a.value, e.value
+
+    def memory_bytes(self):
+        """Device bytes the context holds (scratch, backtrace spill, lists)."""
+        b = _sz()
+        _check(lib().ewfa_context_memory(self.handle, ctypes.byref(b)))
+        return b.value
 
     def max_distance(self, max_len):
         r = lib().ewfa_max_distance(self.handle, max_len)
@@ -422,17 +456,21 @@ class Context:
 class ServerJob:
     """A submitted job: owns its host buffers until it completes."""
 
-    def __init__(self, server, words, pairs, n, cigar_bytes):
+    def __init__(self, server, words, pairs, n, cigar_bytes, results=None, cigars=None):
         self.server = server
         self.words = np.ascontiguousarray(words)
         self.pairs = np.ascontiguousarray(pairs[:max(n, 1)])
         self.n = n
-        self.results = np.zeros(max(n, 1), dtype=RESULT_DTYPE)
-        self.cigars = np.zeros(max(cigar_bytes, 4), dtype=np.int8)
+        # caller-provided outputs (e.g. views of pinned memory) or fresh arrays
+        self.results = results if results is not None else np.zeros(max(n, 1), dtype=RESULT_DTYPE)
+        self.cigars = cigars if cigars is not None else np.zeros(max(cigar_bytes, 4), dtype=np.int8)
+        assert self.results.flags.c_contiguous and self.results.size >= n
+        assert self.cigars.flags.c_contiguous and self.cigars.nbytes >= cigar_bytes
         jid = ctypes.c_uint64()
         _check(lib().ewfa_server_submit(server.handle, self.words.ctypes.data, self.words.size,
                                         self.pairs.ctypes.data, n, self.results.ctypes.data,
-                                        self.cigars.ctypes.data, self.cigars.size, ctypes.byref(jid)))
+                                        self.cigars.ctypes.data, min(self.cigars.nbytes, max(cigar_bytes, 4)),
+                                        ctypes.byref(jid)))
         self.id = jid.value
 
     def done(self):
@@ -451,18 +489,19 @@ class ServerJob:
 
 
 class Server:
-    """The job server (continuous admission by arena footprint)."""
+    """The job server: continuous batching of queued jobs into persistent
+    launches, admitted first-come first-served by arena footprint."""
 
-    def __init__(self, device=0, slots=4, arena_bytes=0, max_distance=0, policy_path=None):
+    def __init__(self, device=0, slots=4, arena_bytes=0, max_distance=0, policy_path=None, max_batch_pairs=0):
         self._policy = policy_path.encode() if policy_path else None
         o = ServerOpts(device=device, slots=slots, arena_bytes=arena_bytes, max_distance=max_distance,
-                       policy_path=self._policy)
+                       policy_path=self._policy, max_batch_pairs=max_batch_pairs)
         h = _vp()
         _check(lib().ewfa_server_create(ctypes.byref(h), ctypes.byref(o)))
         self.handle = h
 
-    def submit(self, words, pairs, n, cigar_bytes):
-        return ServerJob(self, words, pairs, n, cigar_bytes)
+    def submit(self, words, pairs, n, cigar_bytes, results=None, cigars=None):
+        return ServerJob(self, words, pairs, n, cigar_bytes, results, cigars)
 
     def stats(self):
         st = ServerStats()

@@ -167,37 +167,49 @@ int ewfa_last_timing(ewfa_ctx_t *ctx, float *align_ms, float *emit_ms);
 
 /* Context flags (ewfa_set_flags).  Default: 0 (hipGraph replay of the
  * persistent path, no per-kernel timing). */
-#define EWFA_FLAG_TIMING    1  /* record HIP events around each kernel; the
-                                  persistent path then launches eagerly */
+#define EWFA_FLAG_TIMING    1  /* record HIP events around the align and
+                                  emit kernels (inside the captured graph on
+                                  the persistent path) */
 #define EWFA_FLAG_NO_GRAPH  2  /* launch the persistent path eagerly */
 int ewfa_set_flags(ewfa_ctx_t *ctx, int flags);
+
+/* Device bytes the context holds now: op scratch, backtrace spill buffer,
+ * bucket and deferral lists, control words and the host-pipeline buffers. */
+int ewfa_context_memory(ewfa_ctx_t *ctx, size_t *bytes);
 
 /* Largest distance the device path can resolve for sequences up to max_len. */
 int ewfa_max_distance(ewfa_ctx_t *ctx, uint32_t max_len);
 
 /* ---- serving -------------------------------------------------------------- */
 
-/* A job server (an addition; the reference has no serving path).  Jobs are
- * admitted first-come first-served when a slot (a device context with its own
- * stream and worker thread) is free and the device arena has room for the
- * job's footprint: its packed sequences, descriptors, results and CIGAR
- * streams, which stay resident on the device while it runs, plus its op
- * scratch.  Admitted jobs run concurrently; later arrivals wait in the queue.
- * Every job's results are identical to aligning it alone. */
+/* A job server (an addition; the reference has no serving path) with
+ * continuous batching.  Each slot is a worker thread with its own device
+ * context and stream.  A free slot takes the queued jobs first-come
+ * first-served, as many as fit the device arena together (and at most
+ * max_batch_pairs pairs, a single larger job excepted), and runs them as ONE
+ * persistent launch; jobs that arrive meanwhile form the next batch on the
+ * next free slot, so batches overlap on the device.  A job's footprint in the
+ * arena is its packed sequences and CIGAR streams (resident while it runs)
+ * plus its share of the batch's launch state: descriptors, results and the
+ * traceback op scratch for its pairs at the batch's longest sequence.  Every
+ * job's results are identical to aligning it alone. */
 typedef struct ewfa_server ewfa_server_t;
 
 typedef struct ewfa_server_opts {
     int device;
-    int slots;               /* concurrent jobs (0: 4) */
-    size_t arena_bytes;      /* device arena for resident job data (0: 4 GiB) */
+    int slots;               /* concurrent batches (0: 4) */
+    size_t arena_bytes;      /* device arena for job data and launch state (0: 4 GiB) */
     int max_distance;        /* as ewfa_opts_t */
     const char *policy_path; /* as ewfa_opts_t */
+    size_t max_batch_pairs;  /* pairs merged into one launch (0: 262144) */
 } ewfa_server_opts_t;
 
 typedef struct ewfa_server_stats {
     uint64_t submitted, completed, failed;
     size_t arena_bytes, arena_peak;   /* capacity and peak use */
-    int slots, peak_running;          /* slots and peak concurrent jobs */
+    int slots, peak_running;          /* slots and peak jobs on the device */
+    uint64_t batches;                 /* launches so far */
+    int peak_batch_jobs;              /* most jobs merged into one launch */
 } ewfa_server_stats_t;
 
 int ewfa_server_create(ewfa_server_t **srv, const ewfa_server_opts_t *opts);

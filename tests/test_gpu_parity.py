@@ -160,14 +160,19 @@ def test_device_api_with_library_buffers():
     dp = ewfa.DeviceBuffer.from_array(w.pairs[:w.n])
     dr = ewfa.DeviceBuffer(w.n * 16)
     dc = ewfa.DeviceBuffer(w.cigar_bytes)
-    with ewfa.Context(flags=ewfa.FLAG_TIMING) as ctx:
-        ctx.align_device(dw.ptr, dp.ptr, w.n, w.max_len, dr.ptr, dc.ptr, None)
-        a_ms, e_ms = ctx.last_timing()
-    ewfa.synchronize()
-    assert a_ms > 0 and e_ms > 0
-    res = dr.download(ewfa.RESULT_DTYPE, w.n)
-    cig = dc.download(np.int8)
-    assert_matches_oracle(res, w.pairs, cig, d, st, "device-api")
+    # timing events inside the captured graph, then on eager launches
+    for flags in (ewfa.FLAG_TIMING, ewfa.FLAG_TIMING | ewfa.FLAG_NO_GRAPH):
+        dr.fill_zero()
+        with ewfa.Context(flags=flags) as ctx:
+            for _ in range(2):
+                ctx.align_device(dw.ptr, dp.ptr, w.n, w.max_len, dr.ptr, dc.ptr, None)
+                a_ms, e_ms = ctx.last_timing()
+                assert a_ms > 0 and e_ms > 0, flags
+            assert ctx.memory_bytes() > 0
+        ewfa.synchronize()
+        res = dr.download(ewfa.RESULT_DTYPE, w.n)
+        cig = dc.download(np.int8)
+        assert_matches_oracle(res, w.pairs, cig, d, st, f"device-api flags {flags}")
 
 
 def test_pack_kernel_matches_host():

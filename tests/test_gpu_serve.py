@@ -1,6 +1,7 @@
 """The job server: every job's results are identical alone and in the mix
-(tolerance 0: the work is integer), jobs overlap on the device, and an arena
-smaller than the offered load still admits and completes every job."""
+(tolerance 0: the work is integer), queued jobs are merged into shared
+launches and overlap on the device, and an arena smaller than the offered load
+still admits and completes every job."""
 import numpy as np
 import pytest
 
@@ -43,6 +44,7 @@ def test_each_job_same_alone_and_in_the_mix():
         st = srv.stats()
     assert st["completed"] == len(jobs) and st["failed"] == 0
     assert st["peak_running"] > 1  # admitted jobs ran side by side
+    assert st["batches"] < len(jobs) and st["peak_batch_jobs"] > 1  # merged launches
     for w, (r0, c0), (r1, c1) in zip(jobs, ref, outs):
         assert np.array_equal(r0, r1)
         s0 = ewfa.streams_from(r0, w.pairs, c0)
@@ -56,9 +58,17 @@ def test_each_job_same_alone_and_in_the_mix():
     assert all(np.array_equal(a, b) for a, b in zip(got, streams))
 
 
+def footprint(w):
+    """Arena bytes of a job alone: resident words and CIGAR streams, plus
+    descriptors, results and op scratch for its pairs (256-byte blocks)."""
+    r = lambda x: (x + 255) // 256 * 256
+    ops = w.n * ((w.max_len + 1) // 32 + 1) * 8
+    return r(w.n_words * 8) + r(w.cigar_bytes) + r(w.n * 40) + r(w.n * 16) + r(ops)
+
+
 def test_small_arena_admits_as_memory_frees():
     jobs = make_jobs(12, seed=6, max_pairs=200)
-    need = max(w.n_words * 8 + w.n * 56 + w.cigar_bytes for w in jobs)
+    need = max(footprint(w) for w in jobs)
     arena = int(need * 2.5)  # room for about two jobs at a time
     say(f"case serve arena {arena} bytes, {len(jobs)} jobs")
     with ewfa.Server(slots=4, arena_bytes=arena) as srv:
@@ -70,6 +80,26 @@ def test_small_arena_admits_as_memory_frees():
     for w, (r, c) in zip(jobs, outs):
         r0, c0 = alone(w)
         assert np.array_equal(r0, r)
+
+
+def test_jobs_arriving_during_a_launch_share_the_next():
+    big = ewfa.Workload(ewfa.synth_spec(kind=1, seed=8, len_lo=2000, len_hi=4000, err_lo=0.05, err_hi=0.10),
+                        3000, threads=4)
+    small = make_jobs(10, seed=9, max_pairs=100)
+    say("case serve one slot: a long job, then 10 small ones queued behind it")
+    with ewfa.Server(slots=1) as srv:
+        handles = [srv.submit(big.words, big.pairs, big.n, big.cigar_bytes)]
+        handles += [srv.submit(w.words, w.pairs, w.n, w.cigar_bytes) for w in small]
+        outs = [h.wait() for h in handles]
+        st = srv.stats()
+    assert st["completed"] == 11 and st["failed"] == 0
+    assert st["batches"] <= 3 and st["peak_batch_jobs"] >= 5
+    for w, (r, c) in zip([big] + small, outs):
+        r0, c0 = alone(w)
+        assert np.array_equal(r0, r)
+        s0 = ewfa.streams_from(r0, w.pairs, c0)
+        s1 = ewfa.streams_from(r, w.pairs, c)
+        assert all(np.array_equal(a, b) for a, b in zip(s0, s1))
 
 
 def test_job_larger_than_arena_is_refused():
