@@ -227,10 +227,26 @@ def leg_c4(args, threads):
     return out
 
 
+def serve_layout(w):
+    """Per-pair CIGAR regions of max(plen, tlen)/4 + 16 bytes, dword aligned:
+    enough for every stream of a pair with at most 10% edits.  A stream holds
+    one byte per edit op plus ceil(run/127) per match run, so at most
+    2D + 1 + min(plen, tlen)/127 bytes, and D is at most the edits applied."""
+    import numpy as np
+    cap = ((np.maximum(w.plen, w.tlen).astype(np.int64) // 4 + 16 + 3) // 4) * 4
+    off = np.zeros(w.n, dtype=np.int64)
+    off[1:] = np.cumsum(cap)[:-1]
+    w.pairs["cigar_cap"][:w.n] = cap
+    w.pairs["cigar_off"][:w.n] = off
+    w.cigar_bytes = int(cap.sum())
+
+
 def leg_served(args, threads, batched_rate):
     """Poisson arrivals of jobs into the job server.  A pool of distinct jobs
-    is submitted in rounds (each round a fresh random order); a job is not
-    resubmitted before its previous submission has completed."""
+    (inputs in pinned memory, shared by their submissions) is submitted in a
+    random order; each submission writes into one of `serve_ring` output sets
+    of its job, and waits only if that set is still in flight.  One untimed
+    pass of the pool warms the server first."""
     import numpy as np
     import ewfa
     rng = np.random.default_rng(5)
@@ -239,18 +255,20 @@ def leg_served(args, threads, batched_rate):
     spec = ewfa.synth_spec(kind=1, seed=5, len_lo=100, len_hi=10_000, err_lo=0.01, err_hi=0.10)
     t0 = time.time()
     w = ewfa.Workload(spec, total, threads=threads)
-    log(f"served: pool of {args.serve_pool} jobs, {total} pairs, generated in {time.time() - t0:.1f}s")
+    serve_layout(w)
     words_pin = ewfa.PinnedArray(w.words.size, np.uint64)
     words_pin.array[:] = w.words
-    res_pin = ewfa.PinnedArray(total, ewfa.RESULT_DTYPE)
-    cig_pin = ewfa.PinnedArray(w.cigar_bytes, np.int8)
     jobs = []
     a = 0
     for s in sizes:
         b = a + int(s)
-        wv, sub, cb, _, c0 = ewfa.slice_pairs(words_pin.array, w.pairs, a, b)
-        jobs.append((wv, sub, b - a, cb, res_pin.array[a:b], cig_pin.array[c0:c0 + cb]))
+        wv, sub, cb, _, _ = ewfa.slice_pairs(words_pin.array, w.pairs, a, b)
+        outs = [(np.zeros(b - a, dtype=ewfa.RESULT_DTYPE), np.zeros(cb, dtype=np.int8))
+                for _ in range(args.serve_ring)]
+        jobs.append((wv, sub, b - a, cb, outs, a))
         a = b
+    log(f"served: pool of {len(jobs)} jobs, {total} pairs, {args.serve_ring} output sets each, "
+        f"ready in {time.time() - t0:.1f}s")
     mean_pairs = total / len(jobs)
     lam = args.serve_load * batched_rate / mean_pairs
     k = max(len(jobs), int(args.serve_seconds * lam))
@@ -259,8 +277,17 @@ def leg_served(args, threads, batched_rate):
     log(f"served: lambda {lam:.0f} jobs/s ({args.serve_load:.2f} x batched {batched_rate / 1e6:.2f} M aln/s), "
         f"{k} submissions")
 
-    srv = ewfa.Server(slots=args.serve_slots, arena_bytes=args.serve_arena << 30)
-    inflight = [None] * len(jobs)
+    srv = ewfa.Server(slots=args.serve_slots, arena_bytes=args.serve_arena << 30,
+                      max_batch_pairs=args.serve_batch, max_batch_cigar_bytes=args.serve_batch_bytes << 20)
+    # warm-up: every job once, into its last output set
+    warm = [srv.submit(wv, sub, nb, cb, results=outs[-1][0], cigars=outs[-1][1])
+            for wv, sub, nb, cb, outs, _ in jobs]
+    for h in warm:
+        h.wait()
+    st0 = srv.stats()
+    inflight = [[None] * args.serve_ring for _ in jobs]
+    uses = [0] * len(jobs)
+    last_set = [args.serve_ring - 1] * len(jobs)
     sub_t = np.zeros(k)
     handles = []
     throttled = 0
@@ -273,13 +300,17 @@ def leg_served(args, threads, batched_rate):
                 break
             time.sleep(min(target - now, 0.0005))
         j = int(order[i])
-        if inflight[j] is not None and not inflight[j].done():
+        r = uses[j] % args.serve_ring
+        uses[j] += 1
+        prev = inflight[j][r]
+        if prev is not None and not prev.done():
             throttled += 1
-            inflight[j].wait()
-        wv, sub, nb, cb, ro, co = jobs[j]
+            prev.wait()
+        wv, sub, nb, cb, outs, _ = jobs[j]
         sub_t[i] = time.perf_counter() - t_start
-        h = srv.submit(wv, sub, nb, cb, results=ro, cigars=co)
-        inflight[j] = h
+        h = srv.submit(wv, sub, nb, cb, results=outs[r][0], cigars=outs[r][1])
+        inflight[j][r] = h
+        last_set[j] = r
         handles.append((h, i, nb))
     for h, _, _ in handles:
         h.wait()
@@ -289,19 +320,31 @@ def leg_served(args, threads, batched_rate):
     pairs_done = int(sum(nb for _, _, nb in handles))
     span = float(finish.max() - arrivals[0])
     srv.close()
+    # each job's last outputs, gathered into the pool layout for the checks
+    res = np.zeros(total, dtype=ewfa.RESULT_DTYPE)
+    cig = np.zeros(w.cigar_bytes, dtype=np.int8)
+    for (wv, sub, nb, cb, outs, first), r in zip(jobs, last_set):
+        res[first:first + nb] = outs[r][0]
+        c0 = int(w.pairs[first]["cigar_off"])
+        cig[c0:c0 + cb] = outs[r][1]
+    d = lambda key: st[key] - st0[key]
     out = {"lambda_jobs_per_s": round(lam, 1), "offered_frac_of_batched": args.serve_load,
-           "jobs": k, "pairs": pairs_done, "mean_job_pairs": round(mean_pairs, 1),
+           "offered": round(lam * mean_pairs, 1), "jobs": k, "pairs": pairs_done,
+           "mean_job_pairs": round(mean_pairs, 1),
            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
            "value": round(pairs_done / span, 1), "unit": "alignments/s",
            "frac_of_batched": round(pairs_done / span / batched_rate, 3),
-           "batches": int(st["batches"]), "peak_batch_jobs": int(st["peak_batch_jobs"]),
-           "slots": args.serve_slots, "throttled_resubmits": throttled,
-           "arena_peak_bytes": int(st["arena_peak"]), "failed": int(st["failed"])}
-    # the served outputs of each job's last submission, checked below
-    out["w"], out["res"], out["cig"] = w, res_pin.array.copy(), cig_pin.array.copy()
+           "launches": int(d("batches")), "peak_batch_jobs": int(st["peak_batch_jobs"]),
+           "slots": args.serve_slots, "max_batch_pairs": args.serve_batch, "throttled_resubmits": throttled,
+           "arena_peak_bytes": int(st["arena_peak"]), "failed": int(d("failed")),
+           "pairs_not_ok": int((res["status"] != ewfa.PAIR_OK).sum()),
+           "server_s": {"enqueue": round(d("upload_s"), 3), "device_wait": round(d("align_s"), 3),
+                        "handout": round(d("handout_s"), 3)}}
+    out["w"], out["res"], out["cig"] = w, res, cig
     log(f"served: {out['value'] / 1e6:.2f} M aln/s ({out['frac_of_batched']:.2f} of batched), "
-        f"p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, {out['batches']} launches")
+        f"p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, {out['launches']} launches, "
+        f"{throttled} throttled")
     return out
 
 
@@ -379,9 +422,12 @@ def main():
     ap.add_argument("--c4-pairs", type=int, default=20_000)
     ap.add_argument("--serve-load", type=float, default=0.85, help="offered load / batched c3 rate")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
-    ap.add_argument("--serve-pool", type=int, default=256)
-    ap.add_argument("--serve-slots", type=int, default=3)
-    ap.add_argument("--serve-arena", type=int, default=16, help="GiB")
+    ap.add_argument("--serve-pool", type=int, default=384)
+    ap.add_argument("--serve-ring", type=int, default=6, help="output sets per pool job")
+    ap.add_argument("--serve-slots", type=int, default=1)
+    ap.add_argument("--serve-batch", type=int, default=524288, help="max pairs per launch")
+    ap.add_argument("--serve-batch-bytes", type=int, default=2048, help="max CIGAR MiB per launch")
+    ap.add_argument("--serve-arena", type=int, default=24, help="GiB")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")

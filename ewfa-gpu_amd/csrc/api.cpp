@@ -13,6 +13,7 @@
 
 #include "ewfa_internal.h"
 #include "policy.h"
+#include "runtime.h"
 
 namespace {
 
@@ -48,6 +49,8 @@ struct DevBuf {
     size_t bytes = 0;
     int reserve(size_t n) {
         if (n <= bytes) return EWFA_OK;
+        // queued work may still use the old buffer
+        if (p) (void)hipDeviceSynchronize();
         if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
@@ -114,6 +117,10 @@ struct ewfa_ctx {
     size_t ops_ext_bytes = 0;
     // host pipeline buffers
     DevBuf h_words, h_pairs, h_results, h_cigars;
+    // CIGAR compaction: dense device copy, block sums, pinned host staging
+    DevBuf dense, scan;
+    int8_t* h_stage = nullptr;
+    size_t h_stage_cap = 0;
     size_t resident_words = 0;
 };
 
@@ -222,12 +229,45 @@ int main_pass(ewfa_ctx* c, uint32_t max_len, uint32_t items, bool persistent, Pa
 // The persistent path's plan: one pass per length bucket that pairs up to
 // max_len can fall in, run longest bucket first; a single bucket runs over
 // the identity list without the bucketing kernel.
+// Deferral passes run on the device after the bucket passes: each goes over
+// the pairs the one before deferred (ping-pong lists whose counts the kernels
+// read), with a growing wavefront budget, so the whole plan needs no host
+// round trip.  A deep pass's HBM backtrace slots are bounded by giving it a
+// smaller grid: it only ever runs the few pairs that get that far.
+constexpr int MAX_CHAIN = 8;
+constexpr size_t CHAIN_BT_BUDGET = (size_t)1 << 30;
+
+struct Chain {
+    int np = 0;
+    Pass pass[MAX_CHAIN];
+};
+
 struct Plan {
     int nb = 0;
     ewfa::BucketTable table{};
     Pass pass[ewfa::MAX_BUCKETS];
     uint32_t blen[ewfa::MAX_BUCKETS];
+    Chain chain;
 };
+
+int next_pass(ewfa_ctx* c, uint32_t max_len, int prev_dcap, uint32_t items, Pass& p);
+
+void plan_chain(ewfa_ctx* c, uint32_t max_len, int dcap_min, uint32_t n, Chain& ch) {
+    ch = Chain{};
+    int dcap = dcap_min;
+    while (ch.np < MAX_CHAIN) {
+        Pass p;
+        if (next_pass(c, max_len, dcap, n, p) != EWFA_OK) break;  // no larger budget fits
+        if (!p.bt_lds) {
+            const size_t slot = 16u * (size_t)ewfa::bt_chunks((uint32_t)p.dcap);
+            const size_t most = std::max<size_t>(CHAIN_BT_BUDGET / slot, 1);
+            if (p.grid > most) p.grid = (uint32_t)most;
+        }
+        ch.pass[ch.np++] = p;
+        dcap = p.dcap;
+    }
+    g_err.clear();
+}
 
 int plan_persistent(ewfa_ctx* c, uint32_t max_len, uint32_t n, Plan& pl) {
     pl = Plan{};
@@ -240,6 +280,9 @@ int plan_persistent(ewfa_ctx* c, uint32_t max_len, uint32_t n, Plan& pl) {
         if (max_len <= b.len_hi) break;
     }
     pl.table.nb = pl.nb;
+    int dcap_min = pl.pass[0].dcap;
+    for (int b = 1; b < pl.nb; ++b) dcap_min = std::min(dcap_min, pl.pass[b].dcap);
+    plan_chain(c, max_len, dcap_min, n, pl.chain);
     return EWFA_OK;
 }
 
@@ -288,6 +331,7 @@ int reserve_bt(ewfa_ctx* c, const Pass& p) {
 
 int reserve_plan(ewfa_ctx* c, const Plan& pl, size_t n) {
     for (int b = 0; b < pl.nb; ++b) CHK(reserve_bt(c, pl.pass[b]));
+    for (int k = 0; k < pl.chain.np; ++k) CHK(reserve_bt(c, pl.chain.pass[k]));
     if (pl.nb > 1) CHK(c->buckets.reserve((size_t)pl.nb * std::max<size_t>(n, 1) * 4));
     return EWFA_OK;
 }
@@ -378,6 +422,44 @@ int enqueue_main(ewfa_ctx* c, const Pass& p, const uint64_t* words, const ewfa_p
 
 // Enqueues a persistent plan: bucketing, one align pass per bucket (longest
 // first), the emit per bucket, and the deferred count's copy to the host.
+// The deferral chain of a plan, then the marking of pairs no pass could
+// resolve.
+int enqueue_chain(ewfa_ctx* c, const Chain& ch, const uint64_t* words, const ewfa_pair_t* pairs, uint32_t n,
+                  uint32_t max_len, ewfa_result_t* results, int8_t* cigars, hipStream_t s) {
+    uint32_t* ctrl = c->ctrl.as<uint32_t>();
+    uint32_t* lists[2] = {c->list_a.as<uint32_t>(), c->list_b.as<uint32_t>()};
+    const int ctr[2] = {CTRL_DEFER_A, CTRL_DEFER_B};
+    int cur = 0;
+    for (int k = 0; k < ch.np; ++k) {
+        const Pass& p = ch.pass[k];
+        HIPCHK(ewfa::launch_fill_u32(ctrl + CTRL_QUEUE, 0u, 1, s));
+        HIPCHK(ewfa::launch_fill_u32(ctrl + ctr[1 - cur], 0u, 1, s));
+        ewfa::AlignArgs a = base_args(c, words, pairs, results, max_len);
+        fill_pass_args(a, p, c);
+        a.list = lists[cur];
+        a.count = n;
+        a.count_ptr = ctrl + ctr[cur];
+        a.queue = ctrl + CTRL_QUEUE;
+        a.defer_list = lists[1 - cur];
+        a.defer_count = ctrl + ctr[1 - cur];
+        CHK(launch_pass(a, p, s));
+        ewfa::EmitArgs e{};
+        e.words = words;
+        e.pairs = pairs;
+        e.results = results;
+        e.ops = ops_base(c);
+        e.ops_stride = ewfa::ops_words(max_len + 1);
+        e.cigars = cigars;
+        e.list = lists[cur];
+        e.count = n;
+        e.count_ptr = ctrl + ctr[cur];
+        HIPCHK(ewfa::launch_emit(e, n, s));
+        cur = 1 - cur;
+    }
+    HIPCHK(ewfa::launch_mark_too_distant(lists[cur], ctrl + ctr[cur], results, n, s));
+    return EWFA_OK;
+}
+
 // phase: PLAN_ALL, or PLAN_ALIGN (bucketing and align passes) then PLAN_EMIT
 // (emits and the count's copy) as two graphs timed between their launches.
 enum { PLAN_ALL = 3, PLAN_ALIGN = 1, PLAN_EMIT = 2 };
@@ -421,8 +503,8 @@ int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_
         e.count_ptr = bucketed ? ctrl + CTRL_BCOUNT + b : nullptr;
         CHK(emit(c, e, n, pl.blen[b], true, s));
     }
+    CHK(enqueue_chain(c, pl.chain, words, pairs, n, max_len, results, cigars, s));
     if (timing) HIPCHK(hipEventRecord(c->ev_e1, s));
-    HIPCHK(hipMemcpyAsync(c->host_ctrl + CTRL_DEFER_A, ctrl + CTRL_DEFER_A, 4, hipMemcpyDeviceToHost, s));
     return EWFA_OK;
 }
 
@@ -504,6 +586,61 @@ int align_device_with_scratch(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa
     c->ops_ext = nullptr;
     c->ops_ext_bytes = 0;
     return rc;
+}
+
+hipStream_t context_stream(ewfa_ctx_t* c) { return c ? c->stream : nullptr; }
+
+int enqueue_align(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t* d_pairs, size_t n, uint32_t max_len,
+                  ewfa_result_t* d_results, int8_t* d_cigars, uint64_t* d_ops, size_t ops_bytes) {
+    if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    if (c->opts.mode != EWFA_MODE_PERSISTENT) return fail(EWFA_ERR_STATE, "enqueue needs the persistent mode");
+    if (n == 0) return EWFA_OK;
+    if (!d_words || !d_pairs || !d_results || !d_cigars || !d_ops) return fail(EWFA_ERR_ARG, "null buffer");
+    if (n > 0xFFFFFFF0u) return fail(EWFA_ERR_RANGE, "too many pairs in one call");
+    CHK(set_device(c));
+    c->ops_ext = d_ops;
+    c->ops_ext_bytes = ops_bytes;
+    Plan pl;
+    int rc = reserve_scratch(c, n, max_len);
+    if (rc == EWFA_OK) rc = plan_persistent(c, max_len, (uint32_t)n, pl);
+    if (rc == EWFA_OK) rc = reserve_plan(c, pl, n);
+    if (rc == EWFA_OK)
+        rc = enqueue_plan(c, pl, d_words, d_pairs, (uint32_t)n, max_len, d_results, d_cigars, c->stream, false);
+    c->ops_ext = nullptr;
+    c->ops_ext_bytes = 0;
+    return rc;
+}
+
+int enqueue_compact(ewfa_ctx_t* c, const ewfa_pair_t* d_pairs, const ewfa_result_t* d_results,
+                    const int8_t* d_cigars, size_t n, size_t bound, hipStream_t s) {
+    if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    if (n == 0) return EWFA_OK;
+    CHK(set_device(c));
+    const size_t nb = (n + COMPACT_BLOCK - 1) / COMPACT_BLOCK;
+    CHK(c->dense.reserve(std::max<size_t>(bound, 4)));
+    CHK(c->scan.reserve(nb * 16));
+    HIPCHK(launch_compact(d_pairs, d_results, d_cigars, n, c->scan.as<uint64_t>(), c->dense.as<int8_t>(), s));
+    return EWFA_OK;
+}
+
+int fetch_compact(ewfa_ctx_t* c, size_t total, hipStream_t s, const int8_t** out) {
+    if (!c || !out) return fail(EWFA_ERR_ARG, "null argument");
+    if (total > c->dense.bytes) return fail(EWFA_ERR_STATE, "%zu dense bytes, buffer of %zu", total, c->dense.bytes);
+    if (total > c->h_stage_cap) {
+        if (c->h_stage) (void)hipHostFree(c->h_stage);
+        c->h_stage = nullptr;
+        c->h_stage_cap = 0;
+        const size_t cap = std::max<size_t>(total + total / 4, 1 << 20);
+        if (hipHostMalloc((void**)&c->h_stage, cap, hipHostMallocDefault) != hipSuccess) {
+            c->h_stage = nullptr;
+            return fail(EWFA_ERR_NOMEM, "hipHostMalloc(%zu) failed", cap);
+        }
+        c->h_stage_cap = cap;
+    }
+    if (total) HIPCHK(hipMemcpyAsync(c->h_stage, c->dense.p, total, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *out = c->h_stage;
+    return EWFA_OK;
 }
 
 int set_context_policy(ewfa_ctx_t* c, const Policy& p) {
@@ -609,9 +746,10 @@ int ewfa_destroy(ewfa_ctx_t* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     for (DevBuf* b : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->h_words,
-                      &c->h_pairs, &c->h_results, &c->h_cigars})
+                      &c->h_pairs, &c->h_results, &c->h_cigars, &c->dense, &c->scan})
         b->release();
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
     delete c;
     return EWFA_OK;
 }
@@ -685,7 +823,7 @@ int ewfa_context_memory(ewfa_ctx_t* c, size_t* bytes) {
     if (!c || !bytes) return fail(EWFA_ERR_ARG, "null argument");
     size_t b = 0;
     for (const DevBuf* d : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->h_words,
-                            &c->h_pairs, &c->h_results, &c->h_cigars})
+                            &c->h_pairs, &c->h_results, &c->h_cigars, &c->dense, &c->scan})
         b += d->bytes;
     *bytes = b;
     return EWFA_OK;
@@ -789,9 +927,6 @@ int ewfa_align_device(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t*
                 if (hipEventElapsedTime(&ms, c->ev_a1, c->ev_e1) == hipSuccess) c->emit_ms = ms;
             }
         }
-        int dcap_min = pl.pass[0].dcap;
-        for (int b = 1; b < pl.nb; ++b) dcap_min = std::min(dcap_min, pl.pass[b].dcap);
-        CHK(resolve_deferred(c, dcap_min, d_words, d_pairs, 0, max_len, d_results, d_cigars, s));
         return EWFA_OK;
     }
 
@@ -835,9 +970,11 @@ int ewfa_align_resident(ewfa_ctx_t* c, const ewfa_pair_t* h_pairs, size_t n, ewf
     if (!c->h_words.p) return fail(EWFA_ERR_STATE, "no words uploaded");
     const size_t n_words = c->resident_words;
     uint32_t max_len = 0;
+    size_t caps = 0;
     for (size_t i = 0; i < n; ++i) {
         const ewfa_pair_t& p = h_pairs[i];
         max_len = std::max(max_len, std::max(p.plen, p.tlen));
+        caps += p.cigar_cap;
         if (p.pattern_off + ewfa_packed_words(p.plen) - 1 > n_words ||
             p.text_off + ewfa_packed_words(p.tlen) - 1 > n_words)
             return fail(EWFA_ERR_ARG, "pair %zu reads past the packed words", i);
@@ -852,9 +989,15 @@ int ewfa_align_resident(ewfa_ctx_t* c, const ewfa_pair_t* h_pairs, size_t n, ewf
     HIPCHK(hipMemcpyAsync(c->h_pairs.p, h_pairs, n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, s));
     CHK(ewfa_align_device(c, c->h_words.as<uint64_t>(), c->h_pairs.as<ewfa_pair_t>(), n, max_len,
                           c->h_results.as<ewfa_result_t>(), c->h_cigars.as<int8_t>(), s));
+    // only the written CIGAR bytes come back: packed densely on the device,
+    // placed at each pair's offset on the host
+    CHK(ewfa::enqueue_compact(c, c->h_pairs.as<ewfa_pair_t>(), c->h_results.as<ewfa_result_t>(),
+                              c->h_cigars.as<int8_t>(), n, caps, s));
     HIPCHK(hipMemcpyAsync(h_results, c->h_results.p, n * sizeof(ewfa_result_t), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(h_cigars, c->h_cigars.p, cigar_bytes, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    const int8_t* dense = nullptr;
+    CHK(ewfa::fetch_compact(c, ewfa::compact_total(h_pairs, h_results, n), s, &dense));
+    ewfa::scatter_streams(dense, h_pairs, h_results, n, h_cigars, 8);
     return EWFA_OK;
 }
 

@@ -106,13 +106,34 @@ inline int reg_chunks_for(int dcap) {
     return nc <= 4 ? (nc < 1 ? 1 : nc) : 0;
 }
 
+// Bytes of a pair's CIGAR region the emit kernel wrote: whole dwords from
+// cigar_off up to and including the terminator (or up to cigar_cap when the
+// stream filled or overflowed it); none when the emit did not run.
+__host__ __device__ inline uint32_t written_cigar_bytes(const ewfa_result_t& r, const ewfa_pair_t& p) {
+    if (r.status != EWFA_PAIR_OK && r.status != EWFA_PAIR_CAPPED && r.status != EWFA_PAIR_UNDEFINED &&
+        r.status != EWFA_PAIR_OVERFLOW)
+        return 0u;
+    if ((p.cigar_off & 3u) || (p.cigar_cap & 3u)) return 0u;
+    const uint32_t len = r.cigar_len > 0 ? (uint32_t)r.cigar_len : 0u;
+    const uint64_t w = (uint64_t)(len & ~3u) + 4u;
+    return w < p.cigar_cap ? (uint32_t)w : p.cigar_cap;
+}
+
+// Pairs per block of the CIGAR compaction kernels.
+constexpr uint32_t COMPACT_BLOCK = 1024;
+
 // Kernel launchers (kernels.hip).  All asynchronous on `stream`.
 hipError_t launch_align(const AlignArgs& a, int waves, bool persistent, uint32_t grid,
                         size_t lds, hipStream_t stream);
 hipError_t launch_align_reg(const AlignArgs& a, int nc, bool persistent, uint32_t grid, size_t lds,
                             hipStream_t stream);
 int align_reg_blocks_per_cu(int nc, size_t lds);
+// One lane per pair, grid-stride: grid_items bounds the launch (the count may
+// be read on the device).
 hipError_t launch_emit(const EmitArgs& a, uint32_t grid_items, hipStream_t stream);
+// Marks the pairs of list[0..*count_ptr) EWFA_PAIR_TOO_DISTANT.
+hipError_t launch_mark_too_distant(const uint32_t* list, const uint32_t* count_ptr, ewfa_result_t* results,
+                                   uint32_t max_items, hipStream_t stream);
 // Emit with the sequences staged in LDS: sw words per sequence slot, 64 pairs
 // per block (64 * 2 * sw * 8 bytes of LDS).
 hipError_t launch_emit_lds(const EmitArgs& a, uint32_t grid_items, int sw, hipStream_t stream);
@@ -122,6 +143,11 @@ hipError_t launch_pack(const char* ascii, size_t ascii_stride, const uint32_t* l
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
 hipError_t launch_bucketize(const ewfa_pair_t* pairs, uint32_t n, const BucketTable& t, uint32_t* lists,
                             uint32_t* counts, hipStream_t stream);
+// Packs every pair's written CIGAR bytes, in pair order, densely into
+// `dense` (offsets: the exclusive prefix sum of written_cigar_bytes).
+// `scan` holds 2 * ceil(n / COMPACT_BLOCK) words.
+hipError_t launch_compact(const ewfa_pair_t* pairs, const ewfa_result_t* results, const int8_t* cigars, size_t n,
+                          uint64_t* scan, int8_t* dense, hipStream_t stream);
 // Occupancy of the align kernel for a configuration (blocks per CU).
 int align_blocks_per_cu(int waves, size_t lds);
 hipError_t align_set_max_lds(int waves, size_t lds);

@@ -11,6 +11,8 @@
 #include <vector>
 
 #include "ewfa.h"
+#include "ewfa_internal.h"
+#include "runtime.h"
 
 namespace {
 
@@ -31,6 +33,27 @@ void parallel_for(size_t n, int threads, F&& f) {
 }
 
 }  // namespace
+
+namespace ewfa {
+
+size_t compact_total(const ewfa_pair_t* pairs, const ewfa_result_t* results, size_t n) {
+    size_t t = 0;
+    for (size_t i = 0; i < n; ++i) t += written_cigar_bytes(results[i], pairs[i]);
+    return t;
+}
+
+void scatter_streams(const int8_t* dense, const ewfa_pair_t* pairs, const ewfa_result_t* results, size_t n,
+                     int8_t* h_cigars, int threads) {
+    std::vector<size_t> off(n + 1);
+    off[0] = 0;
+    for (size_t i = 0; i < n; ++i) off[i + 1] = off[i] + written_cigar_bytes(results[i], pairs[i]);
+    parallel_for(n, n >= 65536 ? threads : 1, [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i)
+            if (off[i + 1] > off[i]) memcpy(h_cigars + pairs[i].cigar_off, dense + off[i], off[i + 1] - off[i]);
+    });
+}
+
+}  // namespace ewfa
 
 extern "C" {
 

@@ -674,11 +674,17 @@ struct StreamWriter {
     }
 };
 
+__device__ __forceinline__ void emit_one(const EmitArgs& a, uint32_t i);
+
+// Grid-stride over the items, so a launch sized for the largest count can
+// run over a count that is only known on the device.
 __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a) {
-    const uint32_t item = blockIdx.x * 256u + threadIdx.x;
     const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
-    if (item >= count) return;
-    const uint32_t i = a.list ? a.list[item] : item;
+    for (uint32_t item = blockIdx.x * 256u + threadIdx.x; item < count; item += gridDim.x * 256u)
+        emit_one(a, a.list ? a.list[item] : item);
+}
+
+__device__ __forceinline__ void emit_one(const EmitArgs& a, uint32_t i) {
     ewfa_result_t r = a.results[i];
     if (r.status != EWFA_PAIR_OK && r.status != EWFA_PAIR_CAPPED && r.status != EWFA_PAIR_UNDEFINED)
         return;
@@ -871,6 +877,116 @@ __global__ __launch_bounds__(256) void bucketize_kernel(const ewfa_pair_t* pairs
     }
 }
 
+// Pairs still deferred after the last pass the device can stage: their
+// distance is beyond every wavefront budget that fits in LDS.
+__global__ __launch_bounds__(256) void mark_too_distant_kernel(const uint32_t* list, const uint32_t* count_ptr,
+                                                               ewfa_result_t* results) {
+    const uint32_t count = *count_ptr;
+    for (uint32_t item = blockIdx.x * 256u + threadIdx.x; item < count; item += gridDim.x * 256u)
+        results[list[item]] = ewfa_result_t{0, 0, EWFA_PAIR_TOO_DISTANT, 0};
+}
+
+// ---------------------------------------------------------------------------
+// CIGAR compaction: the written bytes of every pair's region, packed densely
+// in pair order, so that only they cross PCIe.  Three launches: per-block
+// byte sums, one block scanning those sums, and a copy in which each block
+// rescans its own pairs and its four waves copy one pair's dwords at a time.
+
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t t = __shfl_up(v, (unsigned)d);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over a 256-thread block; *total gets the block's sum.
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t* ws, uint64_t* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t inc = wave_incl_scan_u64(v);
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    uint64_t pre = 0;
+    for (int q = 0; q < w; ++q) pre += ws[q];
+    *total = ws[0] + ws[1] + ws[2] + ws[3];
+    __syncthreads();
+    return pre + inc - v;
+}
+
+__global__ __launch_bounds__(256) void compact_sums_kernel(const ewfa_pair_t* pairs, const ewfa_result_t* results,
+                                                           size_t n, uint64_t* bsum) {
+    __shared__ uint64_t ws[4];
+    const size_t base = (size_t)blockIdx.x * COMPACT_BLOCK + threadIdx.x * 4u;
+    uint64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + k < n) s += written_cigar_bytes(results[base + k], pairs[base + k]);
+    uint64_t total;
+    (void)block_excl_scan_u64(s, ws, &total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// One block: exclusive offsets of nb block sums, 1024 at a time.
+__global__ __launch_bounds__(256) void compact_scan_kernel(const uint64_t* bsum, uint64_t* boff, uint32_t nb) {
+    __shared__ uint64_t ws[4];
+    uint64_t carry = 0;
+    for (uint32_t c0 = 0; c0 < nb; c0 += 1024u) {
+        const uint32_t b0 = c0 + threadIdx.x * 4u;
+        uint64_t v[4], s = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[k] = (b0 + k < nb) ? bsum[b0 + k] : 0u;
+            s += v[k];
+        }
+        uint64_t total;
+        uint64_t pre = carry + block_excl_scan_u64(s, ws, &total);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (b0 + k < nb) boff[b0 + k] = pre;
+            pre += v[k];
+        }
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(256) void compact_copy_kernel(const ewfa_pair_t* pairs, const ewfa_result_t* results,
+                                                           const int8_t* cigars, size_t n, const uint64_t* boff,
+                                                           int8_t* dense) {
+    __shared__ uint64_t ws[4];
+    __shared__ uint64_t off[COMPACT_BLOCK];
+    __shared__ uint32_t len[COMPACT_BLOCK];
+    const size_t first = (size_t)blockIdx.x * COMPACT_BLOCK;
+    const uint32_t t4 = threadIdx.x * 4u;
+    uint32_t l[4];
+    uint64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const size_t i = first + t4 + k;
+        l[k] = i < n ? written_cigar_bytes(results[i], pairs[i]) : 0u;
+        s += l[k];
+    }
+    uint64_t total;
+    uint64_t pre = boff[blockIdx.x] + block_excl_scan_u64(s, ws, &total);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        off[t4 + k] = pre;
+        len[t4 + k] = l[k];
+        pre += l[k];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t cnt = (uint32_t)min((size_t)COMPACT_BLOCK, n - first);
+    for (uint32_t j = (uint32_t)w; j < cnt; j += 4u) {
+        const uint32_t nd = len[j] >> 2;
+        if (nd == 0) continue;
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(cigars + pairs[first + j].cigar_off);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(dense + off[j]);
+        for (uint32_t q = (uint32_t)lane; q < nd; q += 64u) dst[q] = src[q];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // pack and fill
 
@@ -993,7 +1109,16 @@ int align_blocks_per_cu(int waves, size_t lds) {
 
 hipError_t launch_emit(const EmitArgs& a, uint32_t grid_items, hipStream_t stream) {
     if (grid_items == 0) return hipSuccess;
-    hipLaunchKernelGGL(emit_kernel, dim3((grid_items + 255) / 256), dim3(256), 0, stream, a);
+    const uint32_t blocks = (grid_items + 255) / 256;
+    hipLaunchKernelGGL(emit_kernel, dim3(blocks < 8192u ? blocks : 8192u), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_too_distant(const uint32_t* list, const uint32_t* count_ptr, ewfa_result_t* results,
+                                   uint32_t max_items, hipStream_t stream) {
+    if (max_items == 0) return hipSuccess;
+    const uint32_t blocks = (max_items + 255) / 256 < 1024u ? (max_items + 255) / 256 : 1024u;
+    hipLaunchKernelGGL(mark_too_distant_kernel, dim3(blocks), dim3(256), 0, stream, list, count_ptr, results);
     return hipGetLastError();
 }
 
@@ -1017,6 +1142,20 @@ hipError_t launch_bucketize(const ewfa_pair_t* pairs, uint32_t n, const BucketTa
                             uint32_t* counts, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(bucketize_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, pairs, n, t, lists, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(const ewfa_pair_t* pairs, const ewfa_result_t* results, const int8_t* cigars, size_t n,
+                          uint64_t* scan, int8_t* dense, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const size_t nb = (n + COMPACT_BLOCK - 1) / COMPACT_BLOCK;
+    if (nb > 0xFFFFFFFFu) return hipErrorInvalidValue;
+    uint64_t* bsum = scan;
+    uint64_t* boff = scan + nb;
+    hipLaunchKernelGGL(compact_sums_kernel, dim3((unsigned)nb), dim3(256), 0, stream, pairs, results, n, bsum);
+    hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(256), 0, stream, bsum, boff, (uint32_t)nb);
+    hipLaunchKernelGGL(compact_copy_kernel, dim3((unsigned)nb), dim3(256), 0, stream, pairs, results, cigars, n,
+                       boff, dense);
     return hipGetLastError();
 }
 

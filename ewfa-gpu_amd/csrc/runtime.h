@@ -1,0 +1,50 @@
+// runtime.h -- internal host interfaces shared by the library's translation
+// units (the job server and the host pipeline use them; callers use ewfa.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "ewfa.h"
+
+namespace ewfa {
+
+// Op scratch bytes an align call over n pairs up to max_len bases needs.
+size_t op_scratch_bytes(size_t n, uint32_t max_len);
+
+// ewfa_align_device with the op scratch in caller memory (the job server's
+// arena) instead of the context's own pool.
+int align_device_with_scratch(ewfa_ctx_t* ctx, const uint64_t* d_words, const ewfa_pair_t* d_pairs, size_t n,
+                              uint32_t max_len, ewfa_result_t* d_results, int8_t* d_cigars, void* stream,
+                              uint64_t* d_ops, size_t ops_bytes);
+
+// The context's own stream (its kernels run there).
+hipStream_t context_stream(ewfa_ctx_t* ctx);
+
+// Enqueues the persistent path (bucketing, align passes, deferral chain,
+// emits) on the context's stream and returns without waiting.  The op
+// scratch is caller memory of at least op_scratch_bytes(n, max_len) bytes.
+// Work queued earlier on that stream may still be running: the context's
+// buffers are only replaced after the device drains.
+int enqueue_align(ewfa_ctx_t* ctx, const uint64_t* d_words, const ewfa_pair_t* d_pairs, size_t n, uint32_t max_len,
+                  ewfa_result_t* d_results, int8_t* d_cigars, uint64_t* d_ops, size_t ops_bytes);
+
+// CIGAR compaction after an align call: enqueues the packing of every pair's
+// written CIGAR bytes into the context's dense buffer (`bound` >= the sum of
+// the pairs' cigar_cap).
+int enqueue_compact(ewfa_ctx_t* ctx, const ewfa_pair_t* d_pairs, const ewfa_result_t* d_results,
+                    const int8_t* d_cigars, size_t n, size_t bound, hipStream_t stream);
+
+// Waits for the stream, then copies the first `total` dense bytes into the
+// context's pinned staging; *out points at them until the next call.
+int fetch_compact(ewfa_ctx_t* ctx, size_t total, hipStream_t stream, const int8_t** out);
+
+// Dense bytes of n pairs (host copies of descriptors and results).
+size_t compact_total(const ewfa_pair_t* pairs, const ewfa_result_t* results, size_t n);
+
+// Places each pair's dense bytes at its cigar_off in h_cigars.
+void scatter_streams(const int8_t* dense, const ewfa_pair_t* pairs, const ewfa_result_t* results, size_t n,
+                     int8_t* h_cigars, int threads);
+
+}  // namespace ewfa

@@ -1,8 +1,17 @@
-// serve.cpp -- the job server: continuous batching of alignment jobs.  Each
-// slot (a worker thread with its own context and stream) takes every queued
-// job that fits the device arena, merges them into one persistent launch and
-// returns each job's results; jobs that arrive meanwhile form the next batch
-// on the next free slot, so batches overlap on the device.
+// serve.cpp -- the job server: continuous batching of alignment jobs.
+//
+// Each slot owns a device context, a compute stream and an upload stream, a
+// worker thread and a completion thread, and two "flights" (batches in
+// progress).  The worker takes every queued job that fits the device arena,
+// uploads and rebases them into one descriptor array and enqueues the whole
+// persistent path for the batch (bucketing, align passes, the device-driven
+// deferral chain, emits), then the CIGAR compaction, which writes the written
+// stream bytes straight into the flight's pinned host buffer, and the results
+// copy, all without waiting; then it forms the next batch while the device
+// works.  The completion thread waits for a flight, hands each job its results
+// and streams and frees the batch's arena blocks.  No copy ever waits on a
+// stream other than its own, and the compute stream has the next batch queued
+// behind the current one.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
@@ -13,12 +22,14 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
 
 #include "ewfa.h"
-#include "policy.h"
+#include "ewfa_internal.h"
+#include "runtime.h"
 
 namespace {
 
@@ -97,6 +108,7 @@ struct Job {
     int8_t* cigars;
     size_t cigar_bytes;
     uint32_t max_len;
+    size_t caps;        // sum of the pairs' cigar_cap
     size_t data_bytes;  // resident block: packed words, then CIGAR streams
     size_t data_off = SIZE_MAX;
     Clock::time_point submitted, finished;
@@ -106,19 +118,73 @@ struct Job {
 
 size_t words_block(const Job* j) { return Arena::round(j->n_words * 8); }
 
-// Per-launch state of a batch of n pairs up to max_len bases: merged
-// descriptors, results and the op scratch of the wavefront traceback.
-size_t state_bytes(size_t n, uint32_t max_len) {
-    return Arena::round(n * sizeof(ewfa_pair_t)) + Arena::round(n * sizeof(ewfa_result_t)) +
-           Arena::round(ewfa::op_scratch_bytes(n, max_len));
-}
+// Launch state of a batch of n pairs up to max_len bases: merged
+// descriptors, results, the traceback op scratch and the compaction's block
+// sums.
+struct StateLayout {
+    size_t pairs, results, ops, scan, total;
+    StateLayout(size_t n, uint32_t max_len) {
+        pairs = 0;
+        results = pairs + Arena::round(n * sizeof(ewfa_pair_t));
+        ops = results + Arena::round(n * sizeof(ewfa_result_t));
+        scan = ops + Arena::round(ewfa::op_scratch_bytes(n, max_len));
+        total = scan + Arena::round(16 * ((n + ewfa::COMPACT_BLOCK - 1) / ewfa::COMPACT_BLOCK));
+    }
+};
 
 // Jobs merged into one launch.
 struct Batch {
     std::vector<Job*> jobs;
-    size_t n = 0;
+    size_t n = 0, caps = 0;
     uint32_t max_len = 0;
     size_t state_off = SIZE_MAX, state = 0;
+};
+
+// Pinned host buffer that only grows (never while a copy may use it).
+template <typename T>
+struct Pinned {
+    T* p = nullptr;
+    size_t cap = 0;
+    ~Pinned() {
+        if (p) (void)hipHostFree(p);
+    }
+    bool reserve(size_t n) {
+        if (n <= cap) return true;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(n + n / 4, 1024);
+        if (hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        cap = want;
+        return true;
+    }
+};
+
+// A batch in progress on a slot.
+struct Flight {
+    Batch b;
+    Pinned<ewfa_pair_t> pairs;      // merged, rebased descriptors
+    Pinned<ewfa_result_t> results;  // the batch's results
+    Pinned<int8_t> dense;           // the batch's compacted CIGAR bytes, written by the device
+    int8_t* dense_dev = nullptr;    // the same memory as the device addresses it
+    hipEvent_t uploaded = nullptr, done = nullptr;
+    int rc = EWFA_OK;
+};
+
+struct Slot {
+    ewfa_server* s = nullptr;  // declared in ewfa.h
+    ewfa_ctx_t* ctx = nullptr;
+    hipStream_t cs = nullptr, up = nullptr;
+    Flight flights[2];
+    std::mutex m;
+    std::condition_variable cv;
+    std::deque<Flight*> idle, busy;
+    bool worker_done = false;
+    int init_rc = EWFA_OK;
+    std::thread worker, completer;
 };
 
 }  // namespace
@@ -131,11 +197,12 @@ struct ewfa_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Job*> queue;
     std::unordered_map<uint64_t, Job*> jobs;
-    std::vector<std::thread> workers;
+    std::vector<Slot*> slots;
     uint64_t next_id = 1;
     bool stopping = false;
     int running = 0, peak_running = 0, peak_batch_jobs = 0;
     uint64_t submitted = 0, completed = 0, failed = 0, batches = 0;
+    double upload_s = 0, align_s = 0, handout_s = 0;
 };
 
 namespace {
@@ -147,9 +214,11 @@ namespace {
 bool form_batch(ewfa_server* s, Batch& b) {
     while (!s->queue.empty()) {
         Job* j = s->queue.front();
-        if (!b.jobs.empty() && b.n + j->n > s->opts.max_batch_pairs) break;
+        if (!b.jobs.empty() &&
+            (b.n + j->n > s->opts.max_batch_pairs || b.caps + j->caps > s->opts.max_batch_cigar_bytes))
+            break;
         const uint32_t ml = std::max(b.max_len, j->max_len);
-        const size_t st = state_bytes(b.n + j->n, ml);
+        const size_t st = StateLayout(b.n + j->n, ml).total;
         const size_t off = s->arena.alloc(j->data_bytes);
         if (off == SIZE_MAX) break;
         const size_t probe = s->arena.alloc(st);
@@ -161,48 +230,47 @@ bool form_batch(ewfa_server* s, Batch& b) {
         j->data_off = off;
         b.jobs.push_back(j);
         b.n += j->n;
+        b.caps += j->caps;
         b.max_len = ml;
         s->queue.pop_front();
     }
     if (b.jobs.empty()) return false;
-    b.state = state_bytes(b.n, b.max_len);
+    b.state = StateLayout(b.n, b.max_len).total;
     b.state_off = s->arena.alloc(b.state);  // the last probe of this size succeeded
     return b.state_off != SIZE_MAX;
 }
 
-// Pinned staging for a batch's merged descriptors.
-struct Staging {
-    ewfa_pair_t* p = nullptr;
-    size_t cap = 0;
-    ~Staging() {
-        if (p) (void)hipHostFree(p);
-    }
-    bool reserve(size_t n) {
-        if (n <= cap) return true;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-        if (hipHostMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(ewfa_pair_t), hipHostMallocDefault) != hipSuccess) {
-            p = nullptr;
-            return false;
-        }
-        cap = n;
-        return true;
-    }
-};
+double since(Clock::time_point& t) {
+    const auto now = Clock::now();
+    const double d = std::chrono::duration<double>(now - t).count();
+    t = now;
+    return d;
+}
 
-// Runs a batch as one launch: uploads each job's packed words into its
-// block, rebases its descriptors onto the arena (sequence offsets in words,
-// CIGAR offsets in bytes, both from the arena base), aligns all pairs
-// together and copies each job's results and CIGAR streams back.
-int run_batch(ewfa_server* s, ewfa_ctx_t* ctx, hipStream_t st, Staging& stage, const Batch& b) {
+#define QCHK(x)                                 \
+    do {                                        \
+        if ((x) != hipSuccess) return EWFA_ERR_HIP; \
+    } while (0)
+
+// Enqueues a batch: each job's packed words into its block and the rebased
+// descriptors (sequence offsets in words, CIGAR offsets in bytes, both from
+// the arena base) on the upload stream; then, on the compute stream, the
+// persistent path, the CIGAR compaction and the results copy.  Returns
+// without waiting.
+int enqueue_batch(Slot* sl, Flight* f) {
+    ewfa_server* s = sl->s;
+    const Batch& b = f->b;
     char* base = s->arena.at(0);
-    if (!stage.reserve(b.n)) return EWFA_ERR_NOMEM;
+    if (!f->pairs.reserve(b.n) || !f->results.reserve(b.n)) return EWFA_ERR_NOMEM;
+    if (b.caps > f->dense.cap || !f->dense_dev) {
+        if (!f->dense.reserve(b.caps)) return EWFA_ERR_NOMEM;
+        void* dp = nullptr;
+        QCHK(hipHostGetDevicePointer(&dp, f->dense.p, 0));
+        f->dense_dev = static_cast<int8_t*>(dp);
+    }
     size_t k = 0;
     for (const Job* j : b.jobs) {
-        if (j->n_words &&
-            hipMemcpyAsync(base + j->data_off, j->words, j->n_words * 8, hipMemcpyHostToDevice, st) != hipSuccess)
-            return EWFA_ERR_HIP;
+        if (j->n_words) QCHK(hipMemcpyAsync(base + j->data_off, j->words, j->n_words * 8, hipMemcpyHostToDevice, sl->up));
         const uint64_t woff = j->data_off / 8;
         const uint64_t coff = j->data_off + words_block(j);
         for (size_t i = 0; i < j->n; ++i) {
@@ -210,67 +278,122 @@ int run_batch(ewfa_server* s, ewfa_ctx_t* ctx, hipStream_t st, Staging& stage, c
             p.pattern_off += woff;
             p.text_off += woff;
             p.cigar_off += coff;
-            stage.p[k++] = p;
+            f->pairs.p[k++] = p;
         }
     }
     if (b.n == 0) return EWFA_OK;
-    char* d_pairs = base + b.state_off;
-    char* d_results = d_pairs + Arena::round(b.n * sizeof(ewfa_pair_t));
-    char* d_ops = d_results + Arena::round(b.n * sizeof(ewfa_result_t));
-    if (hipMemcpyAsync(d_pairs, stage.p, b.n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, st) != hipSuccess)
-        return EWFA_ERR_HIP;
-    const int rc = ewfa::align_device_with_scratch(
-        ctx, reinterpret_cast<const uint64_t*>(base), reinterpret_cast<const ewfa_pair_t*>(d_pairs), b.n, b.max_len,
-        reinterpret_cast<ewfa_result_t*>(d_results), reinterpret_cast<int8_t*>(base), st,
-        reinterpret_cast<uint64_t*>(d_ops), ewfa::op_scratch_bytes(b.n, b.max_len));
+    const StateLayout L(b.n, b.max_len);
+    char* st = base + b.state_off;
+    QCHK(hipMemcpyAsync(st + L.pairs, f->pairs.p, b.n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, sl->up));
+    QCHK(hipEventRecord(f->uploaded, sl->up));
+    QCHK(hipStreamWaitEvent(sl->cs, f->uploaded, 0));
+    const ewfa_pair_t* d_pairs = reinterpret_cast<const ewfa_pair_t*>(st + L.pairs);
+    ewfa_result_t* d_results = reinterpret_cast<ewfa_result_t*>(st + L.results);
+    int rc = ewfa::enqueue_align(sl->ctx, reinterpret_cast<const uint64_t*>(base), d_pairs, b.n, b.max_len, d_results,
+                                 reinterpret_cast<int8_t*>(base), reinterpret_cast<uint64_t*>(st + L.ops),
+                                 ewfa::op_scratch_bytes(b.n, b.max_len));
     if (rc != EWFA_OK) return rc;
-    k = 0;
-    for (const Job* j : b.jobs) {
-        if (j->n &&
-            (hipMemcpyAsync(j->results, d_results + k * sizeof(ewfa_result_t), j->n * sizeof(ewfa_result_t),
-                            hipMemcpyDeviceToHost, st) != hipSuccess ||
-             hipMemcpyAsync(j->cigars, base + j->data_off + words_block(j), j->cigar_bytes, hipMemcpyDeviceToHost,
-                            st) != hipSuccess))
-            return EWFA_ERR_HIP;
-        k += j->n;
-    }
-    return hipStreamSynchronize(st) == hipSuccess ? EWFA_OK : EWFA_ERR_HIP;
+    QCHK(ewfa::launch_compact(d_pairs, d_results, reinterpret_cast<const int8_t*>(base), b.n,
+                              reinterpret_cast<uint64_t*>(st + L.scan), f->dense_dev, sl->cs));
+    QCHK(hipMemcpyAsync(f->results.p, d_results, b.n * sizeof(ewfa_result_t), hipMemcpyDeviceToHost, sl->cs));
+    return EWFA_OK;
 }
 
-void worker_main(ewfa_server* s) {
-    ewfa_opts_t o;
-    ewfa_default_opts(&o);
-    o.device = s->opts.device;
-    o.mode = EWFA_MODE_PERSISTENT;
-    o.max_distance = s->opts.max_distance;
-    o.policy_path = s->policy.empty() ? nullptr : s->policy.c_str();
-    ewfa_ctx_t* ctx = nullptr;
-    const int crc = ewfa_create(&ctx, &o);
-    if (crc == EWFA_OK) ewfa_set_flags(ctx, EWFA_FLAG_NO_GRAPH);  // every batch has its own shape
-    hipStream_t st = nullptr;
-    (void)hipSetDevice(s->opts.device);
-    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
-    Staging stage;
+// After the batch's device work (results and compacted CIGAR bytes are in the
+// flight's pinned buffers): each job gets its results and its streams placed
+// at its own offsets.
+int deliver_batch(Flight* f) {
+    const Batch& b = f->b;
+    if (b.n == 0) return EWFA_OK;
+    if (ewfa::compact_total(f->pairs.p, f->results.p, b.n) > b.caps) return EWFA_ERR_STATE;
+    const int8_t* dense = f->dense.p;
+    size_t k = 0;
+    for (Job* j : b.jobs) {
+        memcpy(j->results, f->results.p + k, j->n * sizeof(ewfa_result_t));
+        ewfa::scatter_streams(dense, j->pairs, j->results, j->n, j->cigars, 1);
+        dense += ewfa::compact_total(j->pairs, j->results, j->n);
+        k += j->n;
+    }
+    return EWFA_OK;
+}
+
+void worker_main(Slot* sl) {
+    ewfa_server* s = sl->s;
     for (;;) {
-        Batch b;
+        Flight* f = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(sl->m);
+            sl->cv.wait(lk, [&] { return !sl->idle.empty(); });
+            f = sl->idle.front();
+            sl->idle.pop_front();
+        }
+        f->b = Batch{};
         {
             std::unique_lock<std::mutex> lk(s->m);
             s->cv_work.wait(lk, [&] {
                 if (s->queue.empty()) return s->stopping;
-                return form_batch(s, b);
+                return form_batch(s, f->b);
             });
-            if (b.jobs.empty()) break;  // stopping, queue drained
-            s->running += (int)b.jobs.size();
-            s->peak_running = std::max(s->peak_running, s->running);
-            s->peak_batch_jobs = std::max(s->peak_batch_jobs, (int)b.jobs.size());
-            ++s->batches;
+            if (!f->b.jobs.empty()) {
+                s->running += (int)f->b.jobs.size();
+                s->peak_running = std::max(s->peak_running, s->running);
+                s->peak_batch_jobs = std::max(s->peak_batch_jobs, (int)f->b.jobs.size());
+                ++s->batches;
+            }
+        }
+        if (f->b.jobs.empty()) {  // stopping, queue drained
+            std::lock_guard<std::mutex> lk(sl->m);
+            sl->idle.push_back(f);
+            break;
         }
         s->cv_work.notify_all();  // the next head may fit beside this batch
-        const int rc = (crc != EWFA_OK || !st) ? EWFA_ERR_HIP : run_batch(s, ctx, st, stage, b);
+        auto t = Clock::now();
+        f->rc = sl->init_rc != EWFA_OK ? sl->init_rc : enqueue_batch(sl, f);
+        if (f->rc == EWFA_OK && hipEventRecord(f->done, sl->cs) != hipSuccess) f->rc = EWFA_ERR_HIP;
+        const double up = since(t);
+        {
+            std::lock_guard<std::mutex> lk(s->m);
+            s->upload_s += up;
+        }
+        {
+            std::lock_guard<std::mutex> lk(sl->m);
+            sl->busy.push_back(f);
+        }
+        sl->cv.notify_all();
+    }
+    {
+        std::lock_guard<std::mutex> lk(sl->m);
+        sl->worker_done = true;
+    }
+    sl->cv.notify_all();
+}
+
+void completer_main(Slot* sl) {
+    ewfa_server* s = sl->s;
+    for (;;) {
+        Flight* f = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(sl->m);
+            sl->cv.wait(lk, [&] { return !sl->busy.empty() || sl->worker_done; });
+            if (sl->busy.empty()) break;
+            f = sl->busy.front();
+            sl->busy.pop_front();
+        }
+        auto t = Clock::now();
+        int rc = f->rc;
+        if (rc == EWFA_OK && hipEventSynchronize(f->done) != hipSuccess) rc = EWFA_ERR_HIP;
+        if (rc != EWFA_OK) {
+            // nothing may still use the batch's memory when it is freed
+            (void)hipStreamSynchronize(sl->up);
+            (void)hipStreamSynchronize(sl->cs);
+        }
+        const double al = since(t);
+        if (rc == EWFA_OK) rc = deliver_batch(f);
+        const double ho = since(t);
         {
             std::lock_guard<std::mutex> lk(s->m);
             const auto now = Clock::now();
-            for (Job* j : b.jobs) {
+            for (Job* j : f->b.jobs) {
                 s->arena.free(j->data_off, j->data_bytes);
                 j->rc = rc;
                 j->finished = now;
@@ -278,14 +401,52 @@ void worker_main(ewfa_server* s) {
                 ++s->completed;
                 if (rc != EWFA_OK) ++s->failed;
             }
-            s->arena.free(b.state_off, b.state);
-            s->running -= (int)b.jobs.size();
+            s->arena.free(f->b.state_off, f->b.state);
+            s->running -= (int)f->b.jobs.size();
+            s->align_s += al;
+            s->handout_s += ho;
         }
+        f->b = Batch{};
         s->cv_work.notify_all();
         s->cv_done.notify_all();
+        {
+            std::lock_guard<std::mutex> lk(sl->m);
+            sl->idle.push_back(f);
+        }
+        sl->cv.notify_all();
     }
-    if (st) (void)hipStreamDestroy(st);
-    if (ctx) ewfa_destroy(ctx);
+}
+
+int slot_init(ewfa_server* s, Slot* sl) {
+    ewfa_opts_t o;
+    ewfa_default_opts(&o);
+    o.device = s->opts.device;
+    o.mode = EWFA_MODE_PERSISTENT;
+    o.max_distance = s->opts.max_distance;
+    o.policy_path = s->policy.empty() ? nullptr : s->policy.c_str();
+    int rc = ewfa_create(&sl->ctx, &o);
+    if (rc != EWFA_OK) return rc;
+    ewfa_set_flags(sl->ctx, EWFA_FLAG_NO_GRAPH);
+    sl->cs = ewfa::context_stream(sl->ctx);
+    if (hipSetDevice(s->opts.device) != hipSuccess ||
+        hipStreamCreateWithFlags(&sl->up, hipStreamNonBlocking) != hipSuccess)
+        return EWFA_ERR_HIP;
+    for (Flight& f : sl->flights) {
+        if (hipEventCreateWithFlags(&f.uploaded, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&f.done, hipEventDisableTiming) != hipSuccess)
+            return EWFA_ERR_HIP;
+        sl->idle.push_back(&f);
+    }
+    return EWFA_OK;
+}
+
+void slot_release(Slot* sl) {
+    for (Flight& f : sl->flights) {
+        if (f.uploaded) (void)hipEventDestroy(f.uploaded);
+        if (f.done) (void)hipEventDestroy(f.done);
+    }
+    if (sl->up) (void)hipStreamDestroy(sl->up);
+    if (sl->ctx) ewfa_destroy(sl->ctx);
 }
 
 }  // namespace
@@ -298,10 +459,11 @@ int ewfa_server_create(ewfa_server_t** out, const ewfa_server_opts_t* opts) {
     if (ewfa_device_count() <= 0) return EWFA_ERR_NODEV;
     ewfa_server* s = new ewfa_server();
     if (opts) s->opts = *opts;
-    if (s->opts.slots <= 0) s->opts.slots = 4;
+    if (s->opts.slots <= 0) s->opts.slots = 1;
     if (s->opts.slots > 64) s->opts.slots = 64;
     if (s->opts.arena_bytes == 0) s->opts.arena_bytes = (size_t)4 << 30;
-    if (s->opts.max_batch_pairs == 0) s->opts.max_batch_pairs = (size_t)1 << 18;
+    if (s->opts.max_batch_pairs == 0) s->opts.max_batch_pairs = (size_t)1 << 17;
+    if (s->opts.max_batch_cigar_bytes == 0) s->opts.max_batch_cigar_bytes = (size_t)512 << 20;
     if (s->opts.policy_path) s->policy = s->opts.policy_path;
     s->opts.policy_path = nullptr;
     const int rc = s->arena.init(s->opts.device, s->opts.arena_bytes);
@@ -309,7 +471,16 @@ int ewfa_server_create(ewfa_server_t** out, const ewfa_server_opts_t* opts) {
         delete s;
         return rc;
     }
-    for (int i = 0; i < s->opts.slots; ++i) s->workers.emplace_back(worker_main, s);
+    for (int i = 0; i < s->opts.slots; ++i) {
+        Slot* sl = new Slot();
+        sl->s = s;
+        sl->init_rc = slot_init(s, sl);  // a failed slot fails its batches
+        s->slots.push_back(sl);
+    }
+    for (Slot* sl : s->slots) {
+        sl->worker = std::thread(worker_main, sl);
+        sl->completer = std::thread(completer_main, sl);
+    }
     *out = s;
     return EWFA_OK;
 }
@@ -321,7 +492,12 @@ int ewfa_server_destroy(ewfa_server_t* s) {
         s->stopping = true;
     }
     s->cv_work.notify_all();
-    for (auto& t : s->workers) t.join();
+    for (Slot* sl : s->slots) {
+        sl->worker.join();
+        sl->completer.join();
+        slot_release(sl);
+        delete sl;
+    }
     for (auto& kv : s->jobs) delete kv.second;
     s->arena.release();
     delete s;
@@ -333,9 +509,11 @@ int ewfa_server_submit(ewfa_server_t* s, const uint64_t* h_words, size_t n_words
                        uint64_t* job_id) {
     if (!s || !job_id || (n && (!h_words || !h_pairs || !h_results || !h_cigars))) return EWFA_ERR_ARG;
     uint32_t max_len = 0;
+    size_t caps = 0;
     for (size_t i = 0; i < n; ++i) {
         const ewfa_pair_t& p = h_pairs[i];
         max_len = std::max(max_len, std::max(p.plen, p.tlen));
+        caps += p.cigar_cap;
         if (p.pattern_off + ewfa_packed_words(p.plen) - 1 > n_words ||
             p.text_off + ewfa_packed_words(p.tlen) - 1 > n_words)
             return EWFA_ERR_ARG;
@@ -350,8 +528,9 @@ int ewfa_server_submit(ewfa_server_t* s, const uint64_t* h_words, size_t n_words
     j->cigars = h_cigars;
     j->cigar_bytes = cigar_bytes;
     j->max_len = max_len;
+    j->caps = caps;
     j->data_bytes = Arena::round(n_words * 8) + Arena::round(cigar_bytes);
-    if (j->data_bytes + state_bytes(n, max_len) > s->arena.capacity()) {
+    if (j->data_bytes + StateLayout(n, max_len).total > s->arena.capacity()) {
         delete j;
         return EWFA_ERR_RANGE;
     }
@@ -408,6 +587,9 @@ int ewfa_server_stats(ewfa_server_t* s, ewfa_server_stats_t* out) {
     out->peak_running = s->peak_running;
     out->batches = s->batches;
     out->peak_batch_jobs = s->peak_batch_jobs;
+    out->upload_s = s->upload_s;
+    out->align_s = s->align_s;
+    out->handout_s = s->handout_s;
     return EWFA_OK;
 }
 

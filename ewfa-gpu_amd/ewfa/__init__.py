@@ -53,13 +53,14 @@ class CliArgs(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("arena_bytes", ctypes.c_size_t),
                 ("max_distance", ctypes.c_int), ("policy_path", ctypes.c_char_p),
-                ("max_batch_pairs", ctypes.c_size_t)]
+                ("max_batch_pairs", ctypes.c_size_t), ("max_batch_cigar_bytes", ctypes.c_size_t)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
                 ("arena_bytes", ctypes.c_size_t), ("arena_peak", ctypes.c_size_t), ("slots", ctypes.c_int),
-                ("peak_running", ctypes.c_int), ("batches", ctypes.c_uint64), ("peak_batch_jobs", ctypes.c_int)]
+                ("peak_running", ctypes.c_int), ("batches", ctypes.c_uint64), ("peak_batch_jobs", ctypes.c_int),
+                ("upload_s", ctypes.c_double), ("align_s", ctypes.c_double), ("handout_s", ctypes.c_double)]
 
 
 class PairsFile(ctypes.Structure):
@@ -492,10 +493,12 @@ class Server:
     """The job server: continuous batching of queued jobs into persistent
     launches, admitted first-come first-served by arena footprint."""
 
-    def __init__(self, device=0, slots=4, arena_bytes=0, max_distance=0, policy_path=None, max_batch_pairs=0):
+    def __init__(self, device=0, slots=0, arena_bytes=0, max_distance=0, policy_path=None, max_batch_pairs=0,
+                 max_batch_cigar_bytes=0):
         self._policy = policy_path.encode() if policy_path else None
         o = ServerOpts(device=device, slots=slots, arena_bytes=arena_bytes, max_distance=max_distance,
-                       policy_path=self._policy, max_batch_pairs=max_batch_pairs)
+                       policy_path=self._policy, max_batch_pairs=max_batch_pairs,
+                       max_batch_cigar_bytes=max_batch_cigar_bytes)
         h = _vp()
         _check(lib().ewfa_server_create(ctypes.byref(h), ctypes.byref(o)))
         self.handle = h

@@ -183,25 +183,31 @@ int ewfa_max_distance(ewfa_ctx_t *ctx, uint32_t max_len);
 /* ---- serving -------------------------------------------------------------- */
 
 /* A job server (an addition; the reference has no serving path) with
- * continuous batching.  Each slot is a worker thread with its own device
- * context and stream.  A free slot takes the queued jobs first-come
- * first-served, as many as fit the device arena together (and at most
- * max_batch_pairs pairs, a single larger job excepted), and runs them as ONE
- * persistent launch; jobs that arrive meanwhile form the next batch on the
- * next free slot, so batches overlap on the device.  A job's footprint in the
- * arena is its packed sequences and CIGAR streams (resident while it runs)
- * plus its share of the batch's launch state: descriptors, results and the
- * traceback op scratch for its pairs at the batch's longest sequence.  Every
- * job's results are identical to aligning it alone. */
+ * continuous batching.  Each slot has its own device context, compute and
+ * upload streams and two batches in flight.  A slot with a free flight takes
+ * the queued jobs first-come first-served, as many as fit the device arena
+ * together (and at most max_batch_pairs pairs and max_batch_cigar_bytes of
+ * CIGAR regions, a single larger job excepted), and enqueues them as ONE
+ * persistent launch without waiting; jobs that arrive meanwhile form the next
+ * batch, queued behind it on the device.  Only the written CIGAR bytes cross
+ * PCIe, straight into pinned host memory.  A job's footprint in the arena is
+ * its packed sequences and CIGAR regions (resident while it runs) plus its
+ * share of the batch's launch state: descriptors, results and the traceback
+ * op scratch for its pairs at the batch's longest sequence.  Every job's
+ * results are identical to aligning it alone. */
 typedef struct ewfa_server ewfa_server_t;
 
 typedef struct ewfa_server_opts {
     int device;
-    int slots;               /* concurrent batches (0: 4) */
+    int slots;               /* device contexts, each with two batches in
+                                flight (0: 1) */
     size_t arena_bytes;      /* device arena for job data and launch state (0: 4 GiB) */
     int max_distance;        /* as ewfa_opts_t */
     const char *policy_path; /* as ewfa_opts_t */
-    size_t max_batch_pairs;  /* pairs merged into one launch (0: 262144) */
+    size_t max_batch_pairs;  /* pairs merged into one launch (0: 131072) */
+    size_t max_batch_cigar_bytes; /* their CIGAR regions in bytes (0: 512 MiB);
+                                     each slot keeps two pinned host buffers
+                                     of this size for the compacted streams */
 } ewfa_server_opts_t;
 
 typedef struct ewfa_server_stats {
@@ -210,6 +216,10 @@ typedef struct ewfa_server_stats {
     int slots, peak_running;          /* slots and peak jobs on the device */
     uint64_t batches;                 /* launches so far */
     int peak_batch_jobs;              /* most jobs merged into one launch */
+    /* time summed over batches (seconds, host clock): staging and enqueueing
+     * a batch, waiting for its device work, handing results and streams to
+     * its jobs */
+    double upload_s, align_s, handout_s;
 } ewfa_server_stats_t;
 
 int ewfa_server_create(ewfa_server_t **srv, const ewfa_server_opts_t *opts);

@@ -213,3 +213,25 @@ def test_cigar_overflow_status():
         else:
             assert int(res[i]["status"]) == ewfa.PAIR_OK
             assert cig[8 * i: 8 * i + full].tolist() == st[i].tolist()
+
+
+def test_too_distant_beyond_lds_budget():
+    """A pair whose distance is beyond every wavefront budget LDS can hold for
+    its length ends EWFA_PAIR_TOO_DISTANT; a close pair in the same call is
+    aligned as usual."""
+    rng = np.random.default_rng(21)
+    L = 40000
+    a = bytes(rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), L))
+    b = bytes(rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), L))
+    c = a[:2000]
+    d = c[:1000] + b"A" + c[1000:]
+    say("case too distant: random 40 kbp pair and a one-insertion pair")
+    words, pairs, cb, ml = ewfa.from_sequences([a, c], [b, d])
+    with ewfa.Context() as ctx:
+        most = ctx.max_distance(ml)
+        res, cig = ctx.align_host(words, pairs, 2, cb)
+    assert most < L // 3
+    assert int(res[0]["status"]) == ewfa.PAIR_TOO_DISTANT
+    assert int(res[1]["status"]) == ewfa.PAIR_OK and int(res[1]["distance"]) == 1
+    dist, stream = orc.stream(c, d)
+    assert ewfa.streams_from(res[1:], pairs[1:], cig)[0].tolist() == stream

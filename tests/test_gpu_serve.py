@@ -59,11 +59,13 @@ def test_each_job_same_alone_and_in_the_mix():
 
 
 def footprint(w):
-    """Arena bytes of a job alone: resident words and CIGAR streams, plus
-    descriptors, results and op scratch for its pairs (256-byte blocks)."""
+    """Arena bytes of a job alone: resident words and CIGAR regions, plus
+    descriptors, results, op scratch and compaction block sums (256-byte
+    blocks)."""
     r = lambda x: (x + 255) // 256 * 256
     ops = w.n * ((w.max_len + 1) // 32 + 1) * 8
-    return r(w.n_words * 8) + r(w.cigar_bytes) + r(w.n * 40) + r(w.n * 16) + r(ops)
+    scan = 16 * ((w.n + 1023) // 1024)
+    return r(w.n_words * 8) + r(w.cigar_bytes) + r(w.n * 40) + r(w.n * 16) + r(ops) + r(scan)
 
 
 def test_small_arena_admits_as_memory_frees():
