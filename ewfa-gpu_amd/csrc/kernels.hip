@@ -114,9 +114,7 @@ __device__ __forceinline__ uint64_t win32(const uint32_t* S, int i) {
 
 // Equal leading bases of two 32-base windows, 0..32.
 __device__ __forceinline__ int lead_eq(uint64_t x) {
-    const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
-    const uint32_t z = hi ? (uint32_t)__clz((int)hi) : 32u + (uint32_t)__clz((int)lo);  // __clz(0) = 32
-    return (int)(z >> 1);
+    return __clzll((long long)x) >> 1;  // __clzll(0) = 64: all 32 bases equal
 }
 
 // One 32-base step from pattern position v, text position h (clamped into
@@ -430,18 +428,31 @@ __device__ __forceinline__ void reg_level(int (&o)[NC], int d, int lane, const u
         valid[c] = idx <= 2 * d;
         val[c] = m >> 2;
         op[c] = valid[c] ? (m & 3) : 0;
-        acc[c] = step32o(P, T, plen, tlen, val[c] - (idx - d), val[c], valid[c]);
+        // a valid offset is never left of either sequence start; one past an
+        // end extends by nothing, so only lanes inside both sequences read
+        const int v = val[c] - (idx - d), h = val[c];
+        const bool live = valid[c] && v <= plen && h <= tlen;
+        acc[c] = live ? min(lead_eq(win32o(P, v) ^ win32o(T, h)), min(plen - v, tlen - h)) : 0;
     }
+    // lanes whose 32-base step matched throughout continue; their positions
+    // are inside both sequences (a full step left at least 32 bases), so only
+    // the lanes that stopped need a safe index
 #pragma unroll
     for (int c = 0; c < NA; ++c) {
         int last = acc[c];
         const int k = (c << 6) + lane - d;
+        int h = val[c] + acc[c];
+        int v = h - k;
         while (__ballot(last == 32)) {
             const bool go = last == 32;
-            const int e = step32o(P, T, plen, tlen, val[c] - k + acc[c], val[c] + acc[c], go);
-            acc[c] += go ? e : 0;
-            last = go ? e : 0;
+            const int vi = go ? v : 0, hi = go ? h : 0;
+            const int rem = min(plen - vi, tlen - hi);
+            const int e = go ? min(lead_eq(win32o(P, vi) ^ win32o(T, hi)), rem) : 0;
+            v += e;
+            h += e;
+            last = e;
         }
+        acc[c] = h - val[c];
     }
     uint64_t lo_sel = 0, hi_sel = 0;
 #pragma unroll
