@@ -137,12 +137,6 @@ void* worker(void* arg) {
     }
     const ewfa_pairs_file_t& f = *p.file;
     const uint32_t cap = cigar_capacity(p.seq_len);
-    if (ewfa_upload_words(ctx, f.words, f.n_words) != EWFA_OK) {
-        fprintf(stderr, "%s\n", ewfa_last_error());
-        p.rc = 1;
-        ewfa_destroy(ctx);
-        return nullptr;
-    }
 
     // a printer thread formats batch b while the device aligns batch b+1
     std::mutex m;
@@ -180,8 +174,10 @@ void* worker(void* arg) {
         }
         b.results.resize(b.count);
         b.cigars.resize((size_t)b.count * cap);
-        int rc = ewfa_align_resident(ctx, b.pairs.data(), b.count, b.results.data(), b.cigars.data(),
-                                     b.cigars.size());
+        // each call uploads the words its pairs read, chunk by chunk, while
+        // the chunk before computes
+        int rc = ewfa_align_host(ctx, f.words, f.n_words, b.pairs.data(), b.count, b.results.data(),
+                                 b.cigars.data(), b.cigars.size());
         if (rc == EWFA_OK) {
             // streams longer than the reference's buffer: align again with
             // room for any stream of the pair
@@ -199,7 +195,8 @@ void* worker(void* arg) {
                     total += op[j].cigar_cap;
                 }
                 std::vector<int8_t> oc(total);
-                rc = ewfa_align_resident(ctx, op.data(), op.size(), orr.data(), oc.data(), oc.size());
+                rc = ewfa_align_host(ctx, f.words, f.n_words, op.data(), op.size(), orr.data(), oc.data(),
+                                     oc.size());
                 if (rc == EWFA_OK) {
                     // append the long streams after the batch's buffer
                     const size_t base = b.cigars.size();

@@ -121,6 +121,18 @@ struct ewfa_ctx {
     DevBuf dense, scan;
     int8_t* h_stage = nullptr;
     size_t h_stage_cap = 0;
+    // host pipeline: upload stream, per-chunk upload events, and two stages of
+    // pinned memory the device writes compacted streams and results into
+    hipStream_t up = nullptr;
+    std::vector<hipEvent_t> ev_up;
+    struct Stage {
+        int8_t* dense = nullptr;
+        int8_t* dense_dev = nullptr;
+        size_t dense_cap = 0;
+        ewfa_result_t* res = nullptr;
+        size_t res_cap = 0;
+        hipEvent_t done = nullptr;
+    } stage[2];
     size_t resident_words = 0;
 };
 
@@ -572,6 +584,168 @@ int resolve_deferred(ewfa_ctx* c, int prev_dcap, const uint64_t* words, const ew
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Host pipeline of the persistent path.  The pairs go in chunks (at most
+// PIPE_CHUNK pairs and PIPE_CAPS bytes of CIGAR regions each).  A chunk's
+// packed words and descriptors go up on the upload stream while the chunk
+// before it computes.  Each chunk's written CIGAR bytes are packed by the
+// device straight into pinned host memory, beside its results, and the host
+// places them while the next chunk computes.  Two stages alternate.
+
+constexpr size_t PIPE_CHUNK_MIN = (size_t)1 << 17;
+constexpr size_t PIPE_CHUNKS = 4;
+constexpr size_t PIPE_CAPS = (size_t)256 << 20;
+
+struct Chunk {
+    size_t lo, hi, w_lo, w_hi, caps;
+    uint32_t max_len;
+};
+
+// Pinned host memory the device writes directly: coherent, so kernel stores
+// reach host memory without a cache flush.
+int grow_pinned(void** p, size_t* cap, size_t bytes) {
+    if (bytes <= *cap && *p) return EWFA_OK;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    const size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 20);
+    if (hipHostMalloc(p, want, hipHostMallocCoherent) != hipSuccess) {
+        *p = nullptr;
+        return fail(EWFA_ERR_NOMEM, "hipHostMalloc(%zu) failed", want);
+    }
+    *cap = want;
+    return EWFA_OK;
+}
+
+int pipeline_setup(ewfa_ctx* c, size_t chunks) {
+    if (!c->up) HIPCHK(hipStreamCreateWithFlags(&c->up, hipStreamNonBlocking));
+    while (c->ev_up.size() < chunks) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->ev_up.push_back(e);
+    }
+    for (ewfa_ctx::Stage& st : c->stage)
+        if (!st.done) HIPCHK(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+    return EWFA_OK;
+}
+
+// Enqueues chunk k: its uploads (words only when h_words is given), then on
+// the compute stream the persistent plan, the compaction into the stage's
+// pinned buffer and the results copy.
+int pipeline_enqueue(ewfa_ctx* c, const Chunk& ch, size_t k, const uint64_t* h_words, const ewfa_pair_t* h_pairs) {
+    uint64_t* d_words = c->h_words.as<uint64_t>();
+    ewfa_pair_t* d_pairs = c->h_pairs.as<ewfa_pair_t>();
+    ewfa_result_t* d_results = c->h_results.as<ewfa_result_t>();
+    int8_t* d_cigars = c->h_cigars.as<int8_t>();
+    const size_t n = ch.hi - ch.lo;
+    if (h_words && ch.w_hi > ch.w_lo)
+        HIPCHK(hipMemcpyAsync(d_words + ch.w_lo, h_words + ch.w_lo, (ch.w_hi - ch.w_lo) * 8, hipMemcpyHostToDevice,
+                              c->up));
+    HIPCHK(hipMemcpyAsync(d_pairs + ch.lo, h_pairs + ch.lo, n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, c->up));
+    HIPCHK(hipEventRecord(c->ev_up[k], c->up));
+    HIPCHK(hipStreamWaitEvent(c->stream, c->ev_up[k], 0));
+    ewfa_ctx::Stage& st = c->stage[k & 1];
+    if (ch.caps > st.dense_cap || !st.dense) {
+        CHK(grow_pinned((void**)&st.dense, &st.dense_cap, ch.caps));
+        void* dp = nullptr;
+        HIPCHK(hipHostGetDevicePointer(&dp, st.dense, 0));
+        st.dense_dev = static_cast<int8_t*>(dp);
+    }
+    size_t res_bytes = st.res_cap;
+    CHK(grow_pinned((void**)&st.res, &res_bytes, n * sizeof(ewfa_result_t)));
+    st.res_cap = res_bytes;
+    Plan pl;
+    CHK(reserve_scratch(c, n, ch.max_len));
+    CHK(plan_persistent(c, ch.max_len, (uint32_t)n, pl));
+    CHK(reserve_plan(c, pl, n));
+    CHK(enqueue_plan(c, pl, d_words, d_pairs + ch.lo, (uint32_t)n, ch.max_len, d_results + ch.lo, d_cigars,
+                     c->stream, false));
+    CHK(c->scan.reserve(16 * ((n + ewfa::COMPACT_BLOCK - 1) / ewfa::COMPACT_BLOCK)));
+    HIPCHK(ewfa::launch_compact(d_pairs + ch.lo, d_results + ch.lo, d_cigars, n, c->scan.as<uint64_t>(), st.dense_dev,
+                                c->stream));
+    HIPCHK(hipMemcpyAsync(st.res, d_results + ch.lo, n * sizeof(ewfa_result_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(st.done, c->stream));
+    return EWFA_OK;
+}
+
+int pipeline_complete(ewfa_ctx* c, const Chunk& ch, size_t k, const ewfa_pair_t* h_pairs, ewfa_result_t* h_results,
+                      int8_t* h_cigars) {
+    ewfa_ctx::Stage& st = c->stage[k & 1];
+    HIPCHK(hipEventSynchronize(st.done));
+    const size_t n = ch.hi - ch.lo;
+    memcpy(h_results + ch.lo, st.res, n * sizeof(ewfa_result_t));
+    if (ewfa::compact_total(h_pairs + ch.lo, h_results + ch.lo, n) > ch.caps)
+        return fail(EWFA_ERR_STATE, "compacted streams exceed their regions");
+    ewfa::scatter_streams(st.dense, h_pairs + ch.lo, h_results + ch.lo, n, h_cigars, 8);
+    return EWFA_OK;
+}
+
+// h_words null: the words of the last ewfa_upload_words are resident.  The
+// descriptors are checked while the chunks are formed.
+int run_pipeline(ewfa_ctx* c, const uint64_t* h_words, size_t n_words, const ewfa_pair_t* h_pairs, size_t n,
+                 ewfa_result_t* h_results, int8_t* h_cigars, size_t cigar_bytes) {
+    std::vector<Chunk> chunks;
+    uint32_t max_len = 0;
+    size_t most = 0;
+    const size_t per = std::max(PIPE_CHUNK_MIN, (n + PIPE_CHUNKS - 1) / PIPE_CHUNKS);
+    for (size_t i = 0; i < n;) {
+        Chunk ch{i, i, SIZE_MAX, 0, 0, 0};
+        while (ch.hi < n && ch.hi - ch.lo < per &&
+               (ch.hi == ch.lo || ch.caps + h_pairs[ch.hi].cigar_cap <= PIPE_CAPS)) {
+            const ewfa_pair_t& p = h_pairs[ch.hi];
+            if (p.pattern_off + ewfa_packed_words(p.plen) - 1 > n_words ||
+                p.text_off + ewfa_packed_words(p.tlen) - 1 > n_words)
+                return fail(EWFA_ERR_ARG, "pair %zu reads past the packed words", ch.hi);
+            if ((p.cigar_off & 3) || (p.cigar_cap & 3) || p.cigar_off + p.cigar_cap > cigar_bytes)
+                return fail(EWFA_ERR_ARG, "pair %zu has a bad CIGAR region", ch.hi);
+            ch.w_lo = std::min<size_t>(ch.w_lo, std::min(p.pattern_off, p.text_off));
+            ch.w_hi = std::max<size_t>(ch.w_hi, std::max(p.pattern_off + ewfa_packed_words(p.plen),
+                                                         p.text_off + ewfa_packed_words(p.tlen)));
+            ch.caps += p.cigar_cap;
+            ch.max_len = std::max(ch.max_len, std::max(p.plen, p.tlen));
+            ++ch.hi;
+        }
+        ch.w_hi = std::min(ch.w_hi, n_words);
+        if (ch.w_lo > ch.w_hi) ch.w_lo = ch.w_hi;
+        max_len = std::max(max_len, ch.max_len);
+        most = std::max(most, ch.hi - ch.lo);
+        chunks.push_back(ch);
+        i = ch.hi;
+    }
+    CHK(pipeline_setup(c, chunks.size()));
+    if (h_words) {
+        CHK(c->h_words.reserve(std::max<size_t>(n_words, 1) * 8));
+        c->resident_words = 0;  // only the words these pairs read go up
+    }
+    CHK(c->h_pairs.reserve(n * sizeof(ewfa_pair_t)));
+    CHK(c->h_results.reserve(n * sizeof(ewfa_result_t)));
+    CHK(c->h_cigars.reserve(std::max<size_t>(cigar_bytes, 4)));
+    // size the per-chunk scratch for the largest chunk up front, so that no
+    // buffer is replaced while chunks are in flight
+    CHK(reserve_scratch(c, most, max_len));
+    // earlier work on the context's stream finishes before the uploads land
+    HIPCHK(hipEventRecord(c->ev_in, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->up, c->ev_in, 0));
+    int rc = EWFA_OK;
+    size_t enq = 0, done = 0;
+    for (size_t k = 0; k < chunks.size() && rc == EWFA_OK; ++k) {
+        rc = pipeline_enqueue(c, chunks[k], k, h_words, h_pairs);
+        if (rc == EWFA_OK) ++enq;
+        if (rc == EWFA_OK && k >= 1) {
+            rc = pipeline_complete(c, chunks[k - 1], k - 1, h_pairs, h_results, h_cigars);
+            if (rc == EWFA_OK) ++done;
+        }
+    }
+    if (rc == EWFA_OK && done < enq) rc = pipeline_complete(c, chunks[enq - 1], enq - 1, h_pairs, h_results, h_cigars);
+    if (rc != EWFA_OK) {
+        // leave nothing in flight that uses the context's buffers
+        (void)hipStreamSynchronize(c->up);
+        (void)hipStreamSynchronize(c->stream);
+        return rc;
+    }
+    return EWFA_OK;
+}
+
 namespace ewfa {
 size_t op_scratch_bytes(size_t n, uint32_t max_len) { return n * (size_t)ops_words(max_len + 1) * 8; }
 
@@ -750,6 +924,16 @@ int ewfa_destroy(ewfa_ctx_t* c) {
         b->release();
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->up) {
+        (void)hipStreamSynchronize(c->up);
+        (void)hipStreamDestroy(c->up);
+    }
+    for (hipEvent_t e : c->ev_up) (void)hipEventDestroy(e);
+    for (ewfa_ctx::Stage& st : c->stage) {
+        if (st.dense) (void)hipHostFree(st.dense);
+        if (st.res) (void)hipHostFree(st.res);
+        if (st.done) (void)hipEventDestroy(st.done);
+    }
     delete c;
     return EWFA_OK;
 }
@@ -969,6 +1153,10 @@ int ewfa_align_resident(ewfa_ctx_t* c, const ewfa_pair_t* h_pairs, size_t n, ewf
     if (!h_pairs || !h_results || !h_cigars) return fail(EWFA_ERR_ARG, "null buffer");
     if (!c->h_words.p) return fail(EWFA_ERR_STATE, "no words uploaded");
     const size_t n_words = c->resident_words;
+    if (c->opts.mode == EWFA_MODE_PERSISTENT) {
+        CHK(set_device(c));
+        return run_pipeline(c, nullptr, n_words, h_pairs, n, h_results, h_cigars, cigar_bytes);
+    }
     uint32_t max_len = 0;
     size_t caps = 0;
     for (size_t i = 0; i < n; ++i) {
@@ -1005,9 +1193,13 @@ int ewfa_align_host(ewfa_ctx_t* c, const uint64_t* h_words, size_t n_words, cons
                     size_t n, ewfa_result_t* h_results, int8_t* h_cigars, size_t cigar_bytes) {
     if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
     if (n == 0) return EWFA_OK;
-    if (!h_words) return fail(EWFA_ERR_ARG, "null buffer");
-    CHK(ewfa_upload_words(c, h_words, n_words));
-    return ewfa_align_resident(c, h_pairs, n, h_results, h_cigars, cigar_bytes);
+    if (!h_words || !h_pairs || !h_results || !h_cigars) return fail(EWFA_ERR_ARG, "null buffer");
+    if (c->opts.mode != EWFA_MODE_PERSISTENT) {
+        CHK(ewfa_upload_words(c, h_words, n_words));
+        return ewfa_align_resident(c, h_pairs, n, h_results, h_cigars, cigar_bytes);
+    }
+    CHK(set_device(c));
+    return run_pipeline(c, h_words, n_words, h_pairs, n, h_results, h_cigars, cigar_bytes);
 }
 
 }  // extern "C"

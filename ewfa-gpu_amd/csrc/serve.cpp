@@ -140,7 +140,8 @@ struct Batch {
     size_t state_off = SIZE_MAX, state = 0;
 };
 
-// Pinned host buffer that only grows (never while a copy may use it).
+// Pinned host buffer that only grows (never while a copy may use it);
+// coherent, so kernel stores reach host memory without a cache flush.
 template <typename T>
 struct Pinned {
     T* p = nullptr;
@@ -154,7 +155,7 @@ struct Pinned {
         p = nullptr;
         cap = 0;
         const size_t want = std::max<size_t>(n + n / 4, 1024);
-        if (hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocCoherent) != hipSuccess) {
             p = nullptr;
             return false;
         }

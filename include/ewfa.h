@@ -143,10 +143,14 @@ int ewfa_align_device(ewfa_ctx_t *ctx, const uint64_t *d_words, const ewfa_pair_
                       size_t n, uint32_t max_len, ewfa_result_t *d_results, int8_t *d_cigars,
                       void *stream);
 
-/* Host-buffer pipeline: uploads, aligns batch by batch with copy/compute
- * overlap, and downloads results and streams.  Replaces the Sequences batch
- * loop (src/wavefront.cu:173-330, src/aligner.cu:59-66).  Pairs index
- * h_words / h_cigars as in ewfa_align_device. */
+/* Host-buffer pipeline.  Replaces the Sequences batch loop
+ * (src/wavefront.cu:173-330, src/aligner.cu:59-66).  Pairs index h_words /
+ * h_cigars as in ewfa_align_device.  In the persistent mode the pairs go in
+ * chunks: the words a chunk reads go up while the chunk before computes, and
+ * only the written CIGAR bytes come back (bytes of a region past its stream
+ * are left as they were), after which no words count as resident for
+ * ewfa_align_resident.  In the plain mode: all words up (resident
+ * afterwards), then batch by batch. */
 int ewfa_align_host(ewfa_ctx_t *ctx, const uint64_t *h_words, size_t n_words,
                     const ewfa_pair_t *h_pairs, size_t n, ewfa_result_t *h_results,
                     int8_t *h_cigars, size_t cigar_bytes);

@@ -235,3 +235,30 @@ def test_too_distant_beyond_lds_budget():
     assert int(res[1]["status"]) == ewfa.PAIR_OK and int(res[1]["distance"]) == 1
     dist, stream = orc.stream(c, d)
     assert ewfa.streams_from(res[1:], pairs[1:], cig)[0].tolist() == stream
+
+
+def test_host_pipeline_several_chunks():
+    """ewfa_align_host splits large calls into chunks whose uploads overlap
+    the compute of the chunk before; the results equal the device-resident
+    call on every pair, and a sample equals the oracle."""
+    w = workload_case("c1", 300_000, first=20_000)
+    say("case host pipeline: 300,000 pairs of 150 bp (3 chunks)")
+    with ewfa.Context() as ctx:
+        res, cig = ctx.align_host(w.words, w.pairs, w.n, w.cigar_bytes)
+    dw = ewfa.DeviceBuffer.from_array(w.words)
+    dp = ewfa.DeviceBuffer.from_array(w.pairs[:w.n])
+    dr = ewfa.DeviceBuffer(w.n * 16)
+    dc = ewfa.DeviceBuffer(w.cigar_bytes)
+    with ewfa.Context() as ctx:
+        ctx.align_device(dw.ptr, dp.ptr, w.n, w.max_len, dr.ptr, dc.ptr, None)
+    ewfa.synchronize()
+    res_d = dr.download(ewfa.RESULT_DTYPE, w.n)
+    cig_d = dc.download(np.int8)
+    assert np.array_equal(res, res_d)
+    idx = np.arange(0, w.n, 97)
+    sa = ewfa.streams_from(res[idx], w.pairs[idx], cig)
+    sb = ewfa.streams_from(res_d[idx], w.pairs[idx], cig_d)
+    assert all(np.array_equal(a, b) for a, b in zip(sa, sb))
+    ok, d, sl, st = orc.batch_workload(w, idx=idx[:3000])
+    assert np.array_equal(res["distance"][idx[:3000]], d)
+    assert all(np.array_equal(a, b) for a, b in zip(sa[:3000], st))
