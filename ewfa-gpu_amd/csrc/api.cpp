@@ -77,6 +77,7 @@ enum : int {
     CTRL_QUEUE = 0,     // queue head of the plain/deferral passes
     CTRL_DEFER_A = 1,   // deferred-pair counts (ping-pong)
     CTRL_DEFER_B = 2,
+    CTRL_FALLBACK = 3,  // pairs the fused emit left to the emit kernel
     CTRL_BQUEUE = 8,    // queue heads of the bucket passes (MAX_BUCKETS)
     CTRL_BCOUNT = 16,   // pairs per bucket (MAX_BUCKETS)
     CTRL_WORDS = 64
@@ -93,7 +94,7 @@ struct ewfa_ctx {
     hipEvent_t ev_in = nullptr, ev_a0 = nullptr, ev_a1 = nullptr, ev_e1 = nullptr;
     int num_cus = 0;
     size_t lds_max = 0;
-    DevBuf ops, bt, ctrl, list_a, list_b, buckets;
+    DevBuf ops, bt, ctrl, list_a, list_b, buckets, runs, list_fb;
     uint32_t* host_ctrl = nullptr;  // pinned mirror of the control words
     float align_ms = 0.f, emit_ms = 0.f;
     bool timing = false;
@@ -332,10 +333,17 @@ int reserve_scratch(ewfa_ctx* c, size_t n, uint32_t max_len) {
     }
     CHK(c->list_a.reserve((size_t)std::max<size_t>(n, 1) * 4));
     CHK(c->list_b.reserve((size_t)std::max<size_t>(n, 1) * 4));
+    CHK(c->list_fb.reserve((size_t)std::max<size_t>(n, 1) * 4));
     return EWFA_OK;
 }
 
+// Run scratch of a register pass with the emit fused (bytes per wave).
+uint32_t runs_slot_of(const Pass& p) {
+    return ewfa::runs_slot_bytes((uint32_t)std::min(p.dcap, 32 * p.reg - 1));
+}
+
 int reserve_bt(ewfa_ctx* c, const Pass& p) {
+    if (p.reg) return c->runs.reserve((size_t)runs_slot_of(p) * p.grid);
     if (p.bt_lds) return EWFA_OK;
     const size_t slot = 2u * (size_t)ewfa::bt_chunks((uint32_t)p.dcap);
     return c->bt.reserve(slot * p.grid * 8);
@@ -497,12 +505,24 @@ int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_
             a.defer_list = c->list_a.as<uint32_t>();
             a.defer_count = ctrl + CTRL_DEFER_A;
             a.max_len = pl.blen[b];
+            if (p.reg) {  // the register kernel writes its pairs' streams itself
+                a.runs = c->runs.as<uint8_t>();
+                a.runs_slot = runs_slot_of(p);
+                a.cigars = cigars;
+                a.fallback_list = c->list_fb.as<uint32_t>();
+                a.fallback_count = ctrl + CTRL_FALLBACK;
+            }
             CHK(launch_pass(a, p, s));
         }
         if (timing) HIPCHK(hipEventRecord(c->ev_a1, s));
     }
     if (!(phase & PLAN_EMIT)) return EWFA_OK;
+    bool fused = false;
     for (int b = pl.nb - 1; b >= 0; --b) {
+        if (pl.pass[b].reg) {
+            fused = true;
+            continue;
+        }
         ewfa::EmitArgs e{};
         e.words = words;
         e.pairs = pairs;
@@ -514,6 +534,20 @@ int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_
         e.count = n;
         e.count_ptr = bucketed ? ctrl + CTRL_BCOUNT + b : nullptr;
         CHK(emit(c, e, n, pl.blen[b], true, s));
+    }
+    if (fused) {
+        // paths with a match run of 255 bases or more go through the emit kernel
+        ewfa::EmitArgs e{};
+        e.words = words;
+        e.pairs = pairs;
+        e.results = results;
+        e.ops = ops_base(c);
+        e.ops_stride = ewfa::ops_words(max_len + 1);
+        e.cigars = cigars;
+        e.list = c->list_fb.as<uint32_t>();
+        e.count = n;
+        e.count_ptr = ctrl + CTRL_FALLBACK;
+        HIPCHK(ewfa::launch_emit(e, n, s));
     }
     CHK(enqueue_chain(c, pl.chain, words, pairs, n, max_len, results, cigars, s));
     if (timing) HIPCHK(hipEventRecord(c->ev_e1, s));
@@ -919,8 +953,8 @@ int ewfa_destroy(ewfa_ctx_t* c) {
     for (hipEvent_t e : {c->ev_in, c->ev_a0, c->ev_a1, c->ev_e1})
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
-    for (DevBuf* b : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->h_words,
-                      &c->h_pairs, &c->h_results, &c->h_cigars, &c->dense, &c->scan})
+    for (DevBuf* b : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->runs, &c->list_fb,
+                      &c->h_words, &c->h_pairs, &c->h_results, &c->h_cigars, &c->dense, &c->scan})
         b->release();
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -1006,8 +1040,8 @@ int ewfa_pack_ascii(const char* d_ascii, size_t ascii_stride, const uint32_t* d_
 int ewfa_context_memory(ewfa_ctx_t* c, size_t* bytes) {
     if (!c || !bytes) return fail(EWFA_ERR_ARG, "null argument");
     size_t b = 0;
-    for (const DevBuf* d : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->h_words,
-                            &c->h_pairs, &c->h_results, &c->h_cigars, &c->dense, &c->scan})
+    for (const DevBuf* d : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->runs,
+                            &c->list_fb, &c->h_words, &c->h_pairs, &c->h_results, &c->h_cigars, &c->dense, &c->scan})
         b += d->bytes;
     *bytes = b;
     return EWFA_OK;

@@ -70,7 +70,20 @@ struct AlignArgs {
     int32_t seq_words;        // LDS words per sequence (pattern, text)
     int32_t bt_lds;
     uint32_t max_len;
+    // emit fused into the register kernel: per-wave scratch of match-run
+    // lengths (one byte per diagonal and level, saturating at 255), the CIGAR
+    // base, and the list of pairs left to the emit kernel (a run of 255 or
+    // more on their path)
+    uint8_t* runs;            // null: no fused emit
+    uint32_t runs_slot;       // bytes per wave
+    int8_t* cigars;
+    uint32_t* fallback_list;
+    uint32_t* fallback_count;
 };
+
+// Bytes of a wave's run scratch for levels 0..dmax: level 0 takes one
+// 64-byte row, level d >= 1 its chunks after bt_level_base(d).
+__host__ __device__ inline uint32_t runs_slot_bytes(uint32_t dmax) { return 64u * (1u + bt_chunks(dmax)); }
 
 struct EmitArgs {
     const uint64_t* words;

@@ -214,8 +214,138 @@ __device__ __forceinline__ int slot_bt_word(int seq_words, int dcap) {
 // top, i.e. in at most three consecutive 64-diagonal chunks.  The walk then
 // takes each op bit out of lane j with readlane (scalar work), from LDS or
 // HBM alike.
+// Inclusive prefix sum over the 64 lanes with DPP only: row shifts by 1, 2,
+// 4 and 8 within each row of 16 lanes, then lane 15 and lane 31 broadcast
+// into the rows above.  Every lane of the wave must take part.
+__device__ __forceinline__ int wave_incl_scan_i32(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31 into rows 2, 3
+    return x;
+}
+
+// Stream bytes of a match run of r < 255 bases: floor(r/127) bytes of -127
+// then -(r%127) when nonzero (src/kernels.cu:436-446).
+__device__ __forceinline__ int short_run_bytes(int r) {
+    const int q = (r >= 127) + (r >= 254);
+    return q + (r - 127 * q != 0);
+}
+
+// Writes a short run and an op byte (op 0: none) at pos, bytes at or past
+// cap dropped, as the emit kernel's dword writer leaves them.
+__device__ __forceinline__ void put_run_op(int8_t* out, uint32_t cap, uint32_t pos, int r, int op) {
+    const int q = (r >= 127) + (r >= 254);
+    const int m = r - 127 * q;
+    for (int i = 0; i < q; ++i, ++pos)
+        if (pos < cap) out[pos] = (int8_t)-127;
+    if (m) {
+        if (pos < cap) out[pos] = (int8_t)-m;
+        ++pos;
+    }
+    if (op && pos < cap) out[pos] = (int8_t)(op == OP_DEL ? 'D' : op == OP_SUB ? 'X' : 'I');
+}
+
+// Row offset of (level, flattened diagonal) in a wave's run scratch.
+__device__ __forceinline__ uint32_t run_at(int level, int idx) {
+    return level == 0 ? (uint32_t)idx : 64u * (1u + bt_level_base((uint32_t)level)) + (uint32_t)idx;
+}
+
+// The CIGAR stream of one pair from its traced path, written by the wave
+// that aligned it: step s (op at level s+1) sits in lane s%64 of round s/64;
+// each lane finds its diagonal by a suffix sum of the ops, its match run in
+// the run scratch, and its byte offset by a prefix sum, so the stream and its
+// terminator come out as the emit kernel writes them (src/kernels.cu:398-498).
+// A path with a run of 255 or more bases is left to the emit kernel.
+__device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, const ewfa_pair_t& pr, const uint8_t* R,
+                                           int dist, int start, int status, int tk, int lane) {
+    ewfa_result_t* res = a.results + pidx;
+    if ((pr.cigar_off & 3u) || (pr.cigar_cap & 3u)) {
+        if (lane == 0) *res = ewfa_result_t{dist, 0, EWFA_PAIR_BADARG, 0};
+        return;
+    }
+    // the run scratch and the op words were written by other lanes of this wave
+    __threadfence_block();
+    // op of step s (level s+1) in lane s%64 of round s/64; steps up to dist-1
+    // lie in the words the traceback wrote
+    const uint64_t* ow = a.ops + (size_t)pidx * a.ops_stride;
+    const int opr0 = lane < dist ? (int)((ow[lane >> 5] >> ((lane & 31) << 1)) & 3u) : 0;
+    const int opr1 = 64 + lane < dist ? (int)((ow[(64 + lane) >> 5] >> ((lane & 31) << 1)) & 3u) : 0;
+    const int nsteps = status == EWFA_PAIR_UNDEFINED ? 0 : dist;
+    // step s's diagonal: k_s = tk + (sum of op - 2 over steps s..start-1);
+    // round 1 (steps 64..127) first, its total carried into round 0.  The walk
+    // stops at the first op 0 (a capped path): at most one, in the last round.
+    const int x1 = 64 + lane < start ? opr1 - 2 : 0;
+    const int x1_incl = wave_incl_scan_i32(x1);
+    const int tot1 = (int)__builtin_amdgcn_readlane(x1_incl, 63);
+    int stop = nsteps;
+    {
+        const uint64_t z0 = __ballot(lane < nsteps && opr0 == 0);
+        const uint64_t z1 = __ballot(64 + lane < nsteps && opr1 == 0);
+        if (z0) stop = __ffsll((long long)z0) - 1;
+        else if (z1) stop = 64 + __ffsll((long long)z1) - 1;
+    }
+    // pass 1: runs and byte counts; pass 2 (no run of 255 or more): the bytes
+    bool big = false;
+    int fin = 0;
+    if (status != EWFA_PAIR_UNDEFINED && stop == nsteps) {
+        fin = R[run_at(dist, min(max(tk + dist, 0), 2 * dist))];
+        big = fin >= 255;
+    }
+    int run0 = 0, run1 = 0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int st = 64 * r + lane;
+        const int op = r ? opr1 : opr0;
+        const int x = st < start ? op - 2 : 0;
+        const int incl = r ? 0 : wave_incl_scan_i32(x);  // the whole wave takes part
+        const int k = r ? tk + (tot1 - x1_incl + x1) : tk + tot1 + ((int)__builtin_amdgcn_readlane(incl, 63) - incl + x);
+        if (st < nsteps && st <= stop) {
+            // the path's diagonal at level st lies in [-st, st]; the clamp
+            // only keeps the read inside the scratch row
+            const int run = R[run_at(st, min(max(k + st, 0), 2 * st))];
+            big |= run >= 255;
+            if (r) run1 = run; else run0 = run;
+        }
+    }
+    if (__ballot(big)) {
+        if (lane == 0) {
+            *res = ewfa_result_t{dist, 0, status, 0};
+            a.fallback_list[atomicAdd(a.fallback_count, 1u)] = pidx;
+        }
+        return;
+    }
+    int8_t* out = a.cigars + pr.cigar_off;
+    const uint32_t cap = pr.cigar_cap;
+    uint32_t base = 0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int st = 64 * r + lane;
+        const bool act = st < nsteps && st <= stop;
+        const int op = r ? opr1 : opr0;
+        const int run = r ? run1 : run0;
+        const int nb = act ? short_run_bytes(run) + (op != 0 ? 1 : 0) : 0;
+        const int incl = wave_incl_scan_i32(nb);
+        if (nb) put_run_op(out, cap, base + (uint32_t)(incl - nb), run, op);
+        base += (uint32_t)__builtin_amdgcn_readlane(incl, 63);
+    }
+    const int fin_bytes = (status != EWFA_PAIR_UNDEFINED && stop == nsteps) ? short_run_bytes(fin) : 0;
+    const uint32_t len = base + (uint32_t)fin_bytes;
+    if (lane == 0) {
+        if (fin_bytes) put_run_op(out, cap, base, fin, 0);
+        // terminator: zeros up to the next dword boundary inside the region
+        const uint32_t w = (len & ~3u) + 4u < cap ? (len & ~3u) + 4u : cap;
+        for (uint32_t q = len; q < w; ++q) out[q] = 0;
+        *res = ewfa_result_t{dist, (int32_t)len, len > cap ? EWFA_PAIR_OVERFLOW : status, 0};
+    }
+}
+
+template <bool FUSED>
 __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, const uint64_t* BT, int gtid,
-                                            bool done, int d, int limit, int tk) {
+                                            bool done, int d, int limit, int tk, const ewfa_pair_t* prp = nullptr,
+                                            const uint8_t* R = nullptr) {
     ewfa_result_t* res = a.results + pidx;
     const int tka = tk < 0 ? -tk : tk;
     const int cap = a.max_distance;
@@ -296,6 +426,10 @@ __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, c
             --wi;
             word = 0;
         }
+    }
+    if constexpr (FUSED) {
+        fused_emit(a, pidx, *prp, R, dist, start, status, tk, lane);
+        return;
     }
     if (lane == 0) *res = ewfa_result_t{dist, 0, status, 0};
 }
@@ -397,7 +531,7 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
         nxt = tmp;
     }
 
-    finish_pair(a, pidx, BT, gtid, done, d, limit, tk);
+    finish_pair<false>(a, pidx, BT, gtid, done, d, limit, tk);
 }
 
 // One level of the register wavefront with NA active chunks: candidates for
@@ -405,7 +539,7 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
 // continuation loop only while some lane matched all 32 bases.
 template <int NA, int NC>
 __device__ __forceinline__ void reg_level(int (&o)[NC], int d, int lane, const uint64_t* P, const uint64_t* T,
-                                          int plen, int tlen, uint64_t* BTlev) {
+                                          int plen, int tlen, uint64_t* BTlev, uint8_t* Rlev) {
     int p62[NA], p63[NA];
 #pragma unroll
     for (int c = 0; c + 1 < NA; ++c) {
@@ -454,6 +588,8 @@ __device__ __forceinline__ void reg_level(int (&o)[NC], int d, int lane, const u
         }
         acc[c] = h - val[c];
     }
+#pragma unroll
+    for (int c = 0; c < NA; ++c) Rlev[64 * c + lane] = (uint8_t)min(acc[c], 255);
     uint64_t lo_sel = 0, hi_sel = 0;
 #pragma unroll
     for (int c = 0; c < NA; ++c) {
@@ -552,6 +688,9 @@ __device__ __forceinline__ void align_pair_reg(const AlignArgs& a, uint32_t pidx
     for (int c = 0; c < NC; ++c) o[c] = -1;
     const int o00 = extend_o(P, T, plen, tlen, 0, 0);
     if (lane == 0) o[0] = o00;
+    // run scratch of this wave (the emit is fused)
+    uint8_t* R = a.runs + (size_t)blockIdx.x * a.runs_slot;
+    if (lane == 0) R[0] = (uint8_t)min(o00, 255);
 
     int limit = min(a.dcap, 32 * NC - 1);
     const int cap = a.max_distance;
@@ -561,11 +700,12 @@ __device__ __forceinline__ void align_pair_reg(const AlignArgs& a, uint32_t pidx
     while (!done && d < limit) {
         ++d;
         uint64_t* BTlev = BT + 2 * (size_t)bt_level_base((uint32_t)d);
+        uint8_t* Rlev = R + run_at(d, 0);
         const int nch = (d >> 5) + 1;
-        if (nch == 1) reg_level<1, NC>(o, d, lane, P, T, plen, tlen, BTlev);
-        else if (NC >= 2 && nch == 2) reg_level<(NC >= 2 ? 2 : 1), NC>(o, d, lane, P, T, plen, tlen, BTlev);
-        else if (NC >= 3 && nch == 3) reg_level<(NC >= 3 ? 3 : 1), NC>(o, d, lane, P, T, plen, tlen, BTlev);
-        else if (NC >= 4) reg_level<(NC >= 4 ? 4 : 1), NC>(o, d, lane, P, T, plen, tlen, BTlev);
+        if (nch == 1) reg_level<1, NC>(o, d, lane, P, T, plen, tlen, BTlev, Rlev);
+        else if (NC >= 2 && nch == 2) reg_level<(NC >= 2 ? 2 : 1), NC>(o, d, lane, P, T, plen, tlen, BTlev, Rlev);
+        else if (NC >= 3 && nch == 3) reg_level<(NC >= 3 ? 3 : 1), NC>(o, d, lane, P, T, plen, tlen, BTlev, Rlev);
+        else if (NC >= 4) reg_level<(NC >= 4 ? 4 : 1), NC>(o, d, lane, P, T, plen, tlen, BTlev, Rlev);
         if (tka <= d) {
             const int ti = tk + d;  // flattened index of the target diagonal
             int tv = -1;
@@ -576,11 +716,11 @@ __device__ __forceinline__ void align_pair_reg(const AlignArgs& a, uint32_t pidx
         }
     }
     group_sync<1>();
-    finish_pair(a, pidx, BT, lane, done, d, limit, tk);
+    finish_pair<true>(a, pidx, BT, lane, done, d, limit, tk, &pr, R);
 }
 
 template <int NC>
-__global__ __launch_bounds__(64) void align_reg_kernel(AlignArgs a, int persistent) {
+__global__ __launch_bounds__(64, NC <= 3 ? 8 : 6) void align_reg_kernel(AlignArgs a, int persistent) {
     extern __shared__ uint64_t lds[];
     const int lane = threadIdx.x;
     const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;

@@ -262,3 +262,38 @@ def test_host_pipeline_several_chunks():
     ok, d, sl, st = orc.batch_workload(w, idx=idx[:3000])
     assert np.array_equal(res["distance"][idx[:3000]], d)
     assert all(np.array_equal(a, b) for a, b in zip(sa[:3000], st))
+
+
+def _subst(seq, positions):
+    out = bytearray(seq)
+    for i in positions:
+        out[i] = b"C"[0] if out[i] != b"C"[0] else b"G"[0]
+    return bytes(out)
+
+
+def test_match_runs_across_byte_and_chunk_limits():
+    """Match runs of 126, 127, 128, 253, 254, 255 and more bases: the
+    register kernel writes streams itself for runs up to 254 and leaves
+    longer ones to the emit kernel; both agree with the oracle, in one call."""
+    rng = np.random.default_rng(33)
+    base = lambda n: bytes(rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), n))
+    cases = []
+    for runs in ([126, 127, 128], [253, 254, 40], [255, 20], [254, 254, 254], [1000], [127, 127, 127, 127],
+                 [300, 299], [10, 500, 10]):
+        n = sum(runs) + len(runs) - 1
+        p = base(n)
+        cuts = list(np.cumsum(runs[:-1]) + np.arange(len(runs) - 1))
+        cases.append((p, _subst(p, cuts)))
+    cases.append((base(700), base(700)))  # unrelated: short runs, large distance
+    say(f"case match runs: {len(cases)} pairs")
+    P = [p for p, _ in cases]
+    T = [t for _, t in cases]
+    words, pairs, cb, ml = ewfa.from_sequences(P, T)
+    with ewfa.Context() as ctx:
+        res, cig = ctx.align_host(words, pairs, len(P), cb)
+    streams = ewfa.streams_from(res, pairs, cig)
+    for i, (p, t) in enumerate(cases):
+        d, s = orc.stream(p, t)
+        assert int(res[i]["status"]) in (ewfa.PAIR_OK,), i
+        assert int(res[i]["distance"]) == d, i
+        assert streams[i].tolist() == s, (i, streams[i].tolist()[:20], s[:20])
