@@ -389,6 +389,19 @@ void fill_pass_args(ewfa::AlignArgs& a, const Pass& p, ewfa_ctx* c) {
 // The emit kernel for a pass: sequences staged in LDS when they fit (the
 // persistent path), read from global memory otherwise (and on the plain
 // path, which keeps the reference's one-lane-per-pair global walk).
+// Emit over a short list whose count is known only on the device (deferred
+// pairs, paths with long runs): one wave per pair with its sequences in LDS
+// when they fit, else one lane per pair.  For whole buckets one lane per pair
+// keeps more pairs in flight, which wins on long sequences.
+int emit_list(ewfa_ctx* c, const ewfa::EmitArgs& e, uint32_t items, uint32_t max_len, hipStream_t s) {
+    const int ow = overlapped_words_for(max_len);
+    if ((size_t)2 * ow * 8 <= ewfa::EMIT_WAVE_MAX_LDS)
+        HIPCHK(ewfa::launch_emit_wave(e, items, ow, (uint32_t)c->num_cus * 8u, s));
+    else
+        HIPCHK(ewfa::launch_emit(e, items, s));
+    return EWFA_OK;
+}
+
 int emit(ewfa_ctx* c, const ewfa::EmitArgs& e, uint32_t items, uint32_t max_len, bool staged, hipStream_t s) {
     const int sw = seq_words_for(max_len);
     if (staged && sw <= ewfa::EMIT_LDS_MAX_SW)
@@ -473,7 +486,7 @@ int enqueue_chain(ewfa_ctx* c, const Chain& ch, const uint64_t* words, const ewf
         e.list = lists[cur];
         e.count = n;
         e.count_ptr = ctrl + ctr[cur];
-        HIPCHK(ewfa::launch_emit(e, n, s));
+        CHK(emit_list(c, e, n, max_len, s));
         cur = 1 - cur;
     }
     HIPCHK(ewfa::launch_mark_too_distant(lists[cur], ctrl + ctr[cur], results, n, s));
@@ -547,7 +560,7 @@ int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_
         e.list = c->list_fb.as<uint32_t>();
         e.count = n;
         e.count_ptr = ctrl + CTRL_FALLBACK;
-        HIPCHK(ewfa::launch_emit(e, n, s));
+        CHK(emit_list(c, e, n, max_len, s));
     }
     CHK(enqueue_chain(c, pl.chain, words, pairs, n, max_len, results, cigars, s));
     if (timing) HIPCHK(hipEventRecord(c->ev_e1, s));

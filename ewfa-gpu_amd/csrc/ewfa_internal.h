@@ -144,6 +144,11 @@ int align_reg_blocks_per_cu(int nc, size_t lds);
 // One lane per pair, grid-stride: grid_items bounds the launch (the count may
 // be read on the device).
 hipError_t launch_emit(const EmitArgs& a, uint32_t grid_items, hipStream_t stream);
+// Emit with one wave per pair and its sequences staged in LDS as overlapped
+// copies of sw words (2 * sw * 8 bytes per block, at most EMIT_WAVE_MAX_LDS);
+// at most max_grid blocks, grid-stride.
+hipError_t launch_emit_wave(const EmitArgs& a, uint32_t grid_items, int sw, uint32_t max_grid, hipStream_t stream);
+constexpr size_t EMIT_WAVE_MAX_LDS = 65536;
 // Marks the pairs of list[0..*count_ptr) EWFA_PAIR_TOO_DISTANT.
 hipError_t launch_mark_too_distant(const uint32_t* list, const uint32_t* count_ptr, ewfa_result_t* results,
                                    uint32_t max_items, hipStream_t stream);

@@ -1028,6 +1028,93 @@ __global__ __launch_bounds__(256) void bucketize_kernel(const ewfa_pair_t* pairs
     }
 }
 
+// Match run from pattern position v, text position h with the whole wave:
+// lane j compares the 32 bases at 32j of each 2,048-base round; the first lane
+// that stops (mismatch or end) gives the run.  Positions past either end give 0.
+__device__ __forceinline__ int wave_lcp(const uint64_t* P, const uint64_t* T, int plen, int tlen, int v, int h,
+                                        int lane) {
+    const int rem = min(plen - v, tlen - h);
+    for (int base = 0; base < rem; base += 2048) {
+        const int off = base + 32 * lane;
+        int e = 32;
+        if (off < rem) e = min(lead_eq(win32o(P, v + off) ^ win32o(T, h + off)), rem - off);
+        const uint64_t stop = __ballot(off < rem && e < 32);
+        if (stop) {
+            const int j = __ffsll((long long)stop) - 1;
+            return base + 32 * j + __builtin_amdgcn_readlane(e, j);
+        }
+    }
+    return rem > 0 ? rem : 0;
+}
+
+// Emit with one wave per pair, for sequences too long to stage 64 pairs in
+// LDS: the wave stages its pair's sequences (overlapped copies, sw words
+// each), walks the ops with wave_lcp for every match run, and lane 0 writes
+// the stream.  Grid-stride over the items.
+__global__ __launch_bounds__(64) void emit_wave_kernel(EmitArgs a, int sw) {
+    extern __shared__ uint64_t lds[];
+    uint64_t* P = lds;
+    uint64_t* T = lds + sw;
+    const int lane = threadIdx.x;
+    const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
+    for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
+        const uint32_t i = a.list ? a.list[item] : item;
+        const ewfa_result_t r = a.results[i];
+        if (r.status != EWFA_PAIR_OK && r.status != EWFA_PAIR_CAPPED && r.status != EWFA_PAIR_UNDEFINED) continue;
+        const ewfa_pair_t pr = a.pairs[i];
+        if ((pr.cigar_off & 3u) || (pr.cigar_cap & 3u)) {
+            if (lane == 0) a.results[i].status = EWFA_PAIR_BADARG;
+            continue;
+        }
+        const int plen = (int)pr.plen, tlen = (int)pr.tlen;
+        if (2 * ((plen + 31) >> 5) + 2 > sw || 2 * ((tlen + 31) >> 5) + 2 > sw) {
+            if (lane == 0) a.results[i].status = EWFA_PAIR_TOO_LONG;
+            continue;
+        }
+        group_sync<1>();  // the previous pair's walk is done with the copies
+        stage_overlapped(P, a.words + pr.pattern_off, plen, lane);
+        stage_overlapped(T, a.words + pr.text_off, tlen, lane);
+        group_sync<1>();
+        StreamWriter w{a.cigars + pr.cigar_off, pr.cigar_cap, 0u, 0u};  // lane 0's
+        if (r.status != EWFA_PAIR_UNDEFINED) {
+            const uint64_t* ow = a.ops + (size_t)i * a.ops_stride;
+            int k = 0, off = 0;
+            uint64_t word = 0;
+            bool stopped = false;
+            for (int st = 0; st < r.distance; ++st) {
+                const int e = wave_lcp(P, T, plen, tlen, off - k, off, lane);
+                if (lane == 0) w.run(e);
+                off += e;
+                if ((st & 31) == 0) word = ow[st >> 5];
+                const int op = (int)((word >> ((st & 31) << 1)) & 3ull);
+                if (op == OP_DEL) {
+                    if (lane == 0) w.put('D');
+                    --k;
+                } else if (op == OP_SUB) {
+                    if (lane == 0) w.put('X');
+                    ++off;
+                } else if (op == OP_INS) {
+                    if (lane == 0) w.put('I');
+                    ++k;
+                    ++off;
+                } else {
+                    stopped = true;  // op 0 ends the walk (src/kernels.cu:471-478)
+                    break;
+                }
+            }
+            if (!stopped) {
+                const int e = wave_lcp(P, T, plen, tlen, off - k, off, lane);
+                if (lane == 0) w.run(e);
+            }
+        }
+        if (lane == 0) {
+            w.finish();
+            a.results[i].cigar_len = (int32_t)w.len;
+            if (w.len > pr.cigar_cap) a.results[i].status = EWFA_PAIR_OVERFLOW;
+        }
+    }
+}
+
 // Pairs still deferred after the last pass the device can stage: their
 // distance is beyond every wavefront budget that fits in LDS.
 __global__ __launch_bounds__(256) void mark_too_distant_kernel(const uint32_t* list, const uint32_t* count_ptr,
@@ -1262,6 +1349,14 @@ hipError_t launch_emit(const EmitArgs& a, uint32_t grid_items, hipStream_t strea
     if (grid_items == 0) return hipSuccess;
     const uint32_t blocks = (grid_items + 255) / 256;
     hipLaunchKernelGGL(emit_kernel, dim3(blocks < 8192u ? blocks : 8192u), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit_wave(const EmitArgs& a, uint32_t grid_items, int sw, uint32_t max_grid, hipStream_t stream) {
+    if (grid_items == 0) return hipSuccess;
+    const size_t lds = (size_t)2 * (size_t)sw * 8;
+    const uint32_t blocks = grid_items < max_grid ? grid_items : max_grid;
+    hipLaunchKernelGGL(emit_wave_kernel, dim3(blocks), dim3(64), lds, stream, a, sw);
     return hipGetLastError();
 }
 
