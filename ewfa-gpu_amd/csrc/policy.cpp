@@ -21,11 +21,11 @@ Policy default_policy() {
     // tuned on 500,000 pairs of log-uniform length 100..10,000 bp at 1..10%
     // edits; longer pairs take a 16-wave block with as many levels as LDS holds
     //              len_hi   waves dcap  bt_lds grab
-    p.buckets.push_back({300u, 1, 31, 1, 16});
+    p.buckets.push_back({300u, 1, 127, 1, 16});
     p.buckets.push_back({700u, 1, 95, 1, 4});
     p.buckets.push_back({1200u, 1, 127, 1, 1});
     p.buckets.push_back({5000u, 2, 512, 0, 1});
-    p.buckets.push_back({10500u, 4, 2048, 0, 1});
+    p.buckets.push_back({10500u, 4, 1024, 0, 1});
     p.buckets.push_back({0xFFFFFFFFu, 16, 1 << 20, 0, 1});
     return p;
 }

@@ -46,8 +46,9 @@ std::vector<ewfa::BucketPolicy> candidates(uint32_t blen) {
     // a block per pair, backtrace in HBM
     if (blen >= 400)
         for (int waves : {2, 4, 8, 16})
-            for (int dcap : {512, 2048, 1 << 20}) {
+            for (int dcap : {384, 512, 768, 1024, 1536, 2048, 1 << 20}) {
                 if (dcap > 4 * dmax && dcap != 512) continue;
+                if (waves == 16 && dcap < 1024) continue;
                 c.push_back({0xFFFFFFFFu, waves, dcap, 0, 1});
             }
     return c;
