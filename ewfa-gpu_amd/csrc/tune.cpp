@@ -46,9 +46,10 @@ std::vector<ewfa::BucketPolicy> candidates(uint32_t blen) {
     // a block per pair, backtrace in HBM
     if (blen >= 400)
         for (int waves : {2, 4, 8, 16})
-            for (int dcap : {384, 512, 768, 1024, 1536, 2048, 1 << 20}) {
-                if (dcap > 4 * dmax && dcap != 512) continue;
+            for (int dcap : {384, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 1 << 20}) {
+                if (dcap > 4 * dmax && dcap != 512 && dcap != (1 << 20)) continue;
                 if (waves == 16 && dcap < 1024) continue;
+                if (dcap > 2048 && dcap < (1 << 20) && blen < 10000) continue;
                 c.push_back({0xFFFFFFFFu, waves, dcap, 0, 1});
             }
     return c;
