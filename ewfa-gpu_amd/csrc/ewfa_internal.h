@@ -81,9 +81,10 @@ struct AlignArgs {
     uint32_t* fallback_count;
 };
 
-// Bytes of a wave's run scratch for levels 0..dmax: level 0 takes one
-// 64-byte row, level d >= 1 its chunks after bt_level_base(d).
-__host__ __device__ inline uint32_t runs_slot_bytes(uint32_t dmax) { return 64u * (1u + bt_chunks(dmax)); }
+// Bytes of a wave's run scratch for levels 0..dmax: level d's 2d+1 diagonals
+// at d*d, so a pair at distance D touches (D+1)^2 bytes; slots are rounded
+// to 128-byte lines so no two waves share one.
+__host__ __device__ inline uint32_t runs_slot_bytes(uint32_t dmax) { return ((dmax + 1) * (dmax + 1) + 127u) & ~127u; }
 
 struct EmitArgs {
     const uint64_t* words;

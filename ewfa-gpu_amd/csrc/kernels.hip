@@ -248,10 +248,8 @@ __device__ __forceinline__ void put_run_op(int8_t* out, uint32_t cap, uint32_t p
     if (op && pos < cap) out[pos] = (int8_t)(op == OP_DEL ? 'D' : op == OP_SUB ? 'X' : 'I');
 }
 
-// Row offset of (level, flattened diagonal) in a wave's run scratch.
-__device__ __forceinline__ uint32_t run_at(int level, int idx) {
-    return level == 0 ? (uint32_t)idx : 64u * (1u + bt_level_base((uint32_t)level)) + (uint32_t)idx;
-}
+// Offset of (level, flattened diagonal 0..2*level) in a wave's run scratch.
+__device__ __forceinline__ uint32_t run_at(int level, int idx) { return (uint32_t)(level * level + idx); }
 
 // The CIGAR stream of one pair from its traced path, written by the wave
 // that aligned it: step s (op at level s+1) sits in lane s%64 of round s/64;
@@ -534,79 +532,188 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
     finish_pair<false>(a, pidx, BT, gtid, done, d, limit, tk);
 }
 
-// One level of the register wavefront with NA active chunks: candidates for
-// every chunk, the first 32-base compare of every chunk back to back, then a
-// continuation loop only while some lane matched all 32 bases.
+// ---- register wavefront (one wave per pair) --------------------------------
+// Word 3 of a raw byte buffer resource on gfx9 parts (32-bit data format, so
+// untyped accesses are in range; bounds checked against the record count).
+constexpr int BUF_DWORD3 = 0x00020000;
+
+// Its LDS copies are little-endian: 32-bit word j holds bases 16j..16j+15,
+// base 16j+r in bits 2r+1..2r with its two bits swapped (the bit reversal of
+// the packed big-endian word).  Only base equality is ever asked of a copy,
+// which the swap keeps.  64-bit word q holds words q (low half) and q+1
+// (high half); a len-base copy takes 2*ceil(len/32) + 2 words, the last two
+// zero.
+
+// Global words g (bases 32m..32m+31) and gn (the next, or 0) -> copy words
+// 2m and 2m+1.
+__device__ __forceinline__ void le_words(uint64_t g, uint64_t gn, uint64_t& w0, uint64_t& w1) {
+    const uint64_t a = __builtin_bitreverse32((uint32_t)(g >> 32));
+    const uint64_t b = __builtin_bitreverse32((uint32_t)g);
+    const uint64_t c = __builtin_bitreverse32((uint32_t)(gn >> 32));
+    w0 = (b << 32) | a;
+    w1 = (c << 32) | b;
+}
+
+// Equal leading bases (0..32) of the 32-base windows at pattern position v
+// and text position h.  A window is one ds_read2_b64 and two funnel shifts,
+// which use the low five bits of 2i themselves.
+__device__ __forceinline__ int eq_le(const uint64_t* P, const uint64_t* T, int v, int h) {
+    const uint64_t* p = P + (v >> 4);
+    const uint64_t* t = T + (h >> 4);
+    const uint64_t p0 = p[0], p1 = p[1], t0 = t[0], t1 = t[1];
+    const uint32_t sv = (uint32_t)v << 1, sh = (uint32_t)h << 1;
+    const uint32_t xl = __builtin_amdgcn_alignbit((uint32_t)(p0 >> 32), (uint32_t)p0, sv) ^
+                        __builtin_amdgcn_alignbit((uint32_t)(t0 >> 32), (uint32_t)t0, sh);
+    const uint32_t xh = __builtin_amdgcn_alignbit((uint32_t)(p1 >> 32), (uint32_t)p1, sv) ^
+                        __builtin_amdgcn_alignbit((uint32_t)(t1 >> 32), (uint32_t)t1, sh);
+    const uint64_t x = ((uint64_t)xh << 32) | xl;
+    const uint32_t z = x ? (uint32_t)__builtin_ctzll(x) : 64u;
+    return (int)(z >> 1);
+}
+
+// Little-endian copy of a len-base sequence from global words g, cooperatively.
+__device__ __forceinline__ void stage_le(uint64_t* S, const uint64_t* g, int len, int lane) {
+    const int nw = (len + 31) >> 5;
+    for (int m = lane; m <= nw; m += 64) {
+        uint64_t w0 = 0, w1 = 0;
+        if (m < nw) le_words(g[m], m + 1 < nw ? g[m + 1] : 0ull, w0, w1);
+        S[2 * m] = w0;
+        S[2 * m + 1] = w1;
+    }
+}
+
+__device__ __forceinline__ int extend_le(const uint64_t* P, const uint64_t* T, int plen, int tlen, int v, int h) {
+    int acc = 0;
+    for (;;) {
+        const int e = min(eq_le(P, T, v + acc, h + acc), min(plen - v - acc, tlen - h - acc));
+        acc += max(e, 0);
+        if (e < 32) break;
+    }
+    return acc;
+}
+
+// One level of the register wavefront with NA active chunks.  o holds
+// offset + 1 per diagonal (0: none), so the lanes a whole-wave DPP shift
+// leaves without a source read 0, i.e. no offset.  Candidates for every
+// chunk, the first 32-base compare of every chunk back to back, then a
+// continuation loop only while some lane matched all 32 bases.  Writes the
+// run of each lane to byte rrow + 64c + lane of the run scratch Rs (lanes
+// past the level's 2d+1 diagonals write 0 into rows of later levels, which
+// those levels overwrite; the buffer's size drops writes past the wave's
+// slot) and chunk c's two op ballots to BTl[2c], BTl[2c+1].  vtop:
+// the valid lanes of the top chunk (those below it are full).  nidx[c]:
+// -(64c + lane).
 template <int NA, int NC>
-__device__ __forceinline__ void reg_level(int (&o)[NC], int d, int lane, const uint64_t* P, const uint64_t* T,
-                                          int plen, int tlen, uint64_t* BTlev, uint8_t* Rlev) {
+__device__ __forceinline__ void reg_level(int (&o)[NC], const int (&nidx)[NC], int d, uint64_t vtop, int lane,
+                                          const uint64_t* P, const uint64_t* T, int plen, int tlen,
+                                          uint64_t* BTl, __amdgpu_buffer_rsrc_t Rs, int rrow) {
     int p62[NA], p63[NA];
 #pragma unroll
     for (int c = 0; c + 1 < NA; ++c) {
         p62[c] = __builtin_amdgcn_readlane(o[c], 62);
         p63[c] = __builtin_amdgcn_readlane(o[c], 63);
     }
-    int val[NA], op[NA], acc[NA];
+    int val[NA], acc[NA], m[NA];
     bool valid[NA];
 #pragma unroll
     for (int c = 0; c < NA; ++c) {
-        const int in63 = c ? p63[c - 1] : -1;
-        const int in62 = c ? p62[c - 1] : -1;
-        const int s1 = __builtin_amdgcn_update_dpp(in63, o[c], 0x138, 0xF, 0xF, false);
-        const int s2 = __builtin_amdgcn_update_dpp(in62, s1, 0x138, 0xF, 0xF, false);
-        const int idx = (c << 6) + lane;
+        int s1, s2;
+        if (c == 0) {
+            s1 = __builtin_amdgcn_update_dpp(0, o[0], 0x138, 0xF, 0xF, true);
+            s2 = __builtin_amdgcn_update_dpp(0, s1, 0x138, 0xF, 0xF, true);
+        } else {
+            s1 = __builtin_amdgcn_update_dpp(p63[c - 1], o[c], 0x138, 0xF, 0xF, false);
+            s2 = __builtin_amdgcn_update_dpp(p62[c - 1], s1, 0x138, 0xF, 0xF, false);
+        }
         // candidates (src/kernels.cu:131-138) packed as 4*value + op, so one
         // max gives the value and DEL(3) > SUB(2) > INS(1) on ties
-        // (src/kernels.cu:141-152)
-        const int m = max(max(o[c] * 4 + OP_DEL, s1 * 4 + (4 + OP_SUB)), s2 * 4 + (4 + OP_INS));
-        valid[c] = idx <= 2 * d;
-        val[c] = m >> 2;
-        op[c] = valid[c] ? (m & 3) : 0;
-        // a valid offset is never left of either sequence start; one past an
-        // end extends by nothing, so only lanes inside both sequences read
-        const int v = val[c] - (idx - d), h = val[c];
-        const bool live = valid[c] && v <= plen && h <= tlen;
-        acc[c] = live ? min(lead_eq(win32o(P, v) ^ win32o(T, h)), min(plen - v, tlen - h)) : 0;
-    }
-    // lanes whose 32-base step matched throughout continue; their positions
-    // are inside both sequences (a full step left at least 32 bases), so only
-    // the lanes that stopped need a safe index
-#pragma unroll
-    for (int c = 0; c < NA; ++c) {
-        int last = acc[c];
-        const int k = (c << 6) + lane - d;
-        int h = val[c] + acc[c];
-        int v = h - k;
-        while (__ballot(last == 32)) {
-            const bool go = last == 32;
-            const int vi = go ? v : 0, hi = go ? h : 0;
-            const int rem = min(plen - vi, tlen - hi);
-            const int e = go ? min(lead_eq(win32o(P, vi) ^ win32o(T, hi)), rem) : 0;
-            v += e;
-            h += e;
-            last = e;
-        }
-        acc[c] = h - val[c];
-    }
-#pragma unroll
-    for (int c = 0; c < NA; ++c) Rlev[64 * c + lane] = (uint8_t)min(acc[c], 255);
-    uint64_t lo_sel = 0, hi_sel = 0;
-#pragma unroll
-    for (int c = 0; c < NA; ++c) {
-        o[c] = valid[c] ? val[c] + acc[c] : -1;
-        const uint64_t lo = __ballot(op[c] & 1);
-        const uint64_t hi = __ballot(op[c] & 2);
-        if (lane == c) {
-            lo_sel = lo;
-            hi_sel = hi;
+        // (src/kernels.cu:141-152); the +1 of the stored offsets carries
+        // through unchanged
+        // (offsets: DEL o-1, SUB s1, INS s2 in stored units, so the max is
+        // the new offset itself)
+        m[c] = max(max(o[c] * 4 + (OP_DEL - 4), s1 * 4 + OP_SUB), s2 * 4 + OP_INS);
+        // lanes 0..2d-64c hold the chunk's diagonals: all of a chunk below
+        // the top one, vtop of the top one
+        valid[c] = c + 1 < NA ? true : __builtin_amdgcn_inverse_ballot_w64(vtop);
+        val[c] = m[c] >> 2;
+        acc[c] = 0;
+        // a valid offset is never left of either sequence start, and at most
+        // d past an end: its reads stay inside the slot, and a position past
+        // an end extends by nothing
+        if (valid[c]) {
+            const int h = val[c], v = h + nidx[c] + d;
+            acc[c] = max(min(eq_le(P, T, v, h), min(plen - v, tlen - h)), 0);
         }
     }
-    // lane c stores chunk c's two ballots
-    if (lane < NA) {
-        uint64_t* q = BTlev + 2 * lane;
-        q[0] = lo_sel;
-        q[1] = hi_sel;
+    // lanes whose first step matched all 32 bases continue; their positions
+    // are inside both sequences
+#pragma unroll
+    for (int c = 0; c < NA; ++c) {
+        if (__builtin_amdgcn_ballot_w64(acc[c] == 32)) {
+            int last = acc[c];
+            int h = val[c] + acc[c];
+            int v = h + nidx[c] + d;
+            do {
+                if (last == 32) {
+                    last = min(eq_le(P, T, v, h), min(plen - v, tlen - h));
+                    h += last;
+                    v += last;
+                }
+            } while (__builtin_amdgcn_ballot_w64(last == 32));
+            acc[c] = h - val[c];
+        }
     }
+#pragma unroll
+    for (int c = 0; c < NA; ++c) {
+        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)min(acc[c], 255), Rs, lane, rrow + (c << 6), 0);
+        o[c] = valid[c] ? val[c] + 1 + acc[c] : 0;
+    }
+#pragma unroll
+    for (int c = 0; c < NA; ++c) {
+        const uint64_t vm = c + 1 < NA ? ~0ull : vtop;
+        const uint64_t lo = __builtin_amdgcn_ballot_w64((m[c] & 1) != 0) & vm;
+        const uint64_t hi = __builtin_amdgcn_ballot_w64((m[c] & 2) != 0) & vm;
+        if (lane == 0) {
+            BTl[2 * c] = lo;
+            BTl[2 * c + 1] = hi;
+        }
+    }
+}
+
+// Levels d+1..dend with NA chunks, until the target diagonal holds tlen.
+// Returns the last level done, plus REG_HIT when the target was reached
+// there.  Level d's ballots start at word 2*bt_level_base(d) of BT (NA
+// chunks per level in this range), its runs at byte d*d of R.
+constexpr int REG_HIT = 1 << 30;
+
+template <int NA, int NC>
+__device__ __forceinline__ int reg_levels(int (&o)[NC], const int (&nidx)[NC], int d, int dend, int lane,
+                                          const uint64_t* P, const uint64_t* T, int plen, int tlen, int tk,
+                                          uint64_t* BT, __amdgpu_buffer_rsrc_t Rs) {
+    uint64_t* BTl = BT + 2 * (size_t)bt_level_base((uint32_t)d + 1);
+    // valid lanes of the top chunk at level d+1: 2(d+1)+1-64(NA-1), 1..63
+    uint64_t vtop = (1ull << (2 * d + 3 - 64 * (NA - 1))) - 1ull;
+    const int tka = tk < 0 ? -tk : tk;
+    // one exit: stop drops to -1 when the target is reached
+    int stop = dend;
+    do {
+        ++d;
+        reg_level<NA, NC>(o, nidx, d, vtop, lane, P, T, plen, tlen, BTl, Rs, d * d);
+        BTl += 2 * NA;
+        vtop = (vtop << 2) | 3ull;
+        // the target diagonal's offset (flattened index tk + d); before level
+        // |tk| the diagonal is outside the wavefront, and no offset equals -7
+        const int ti = tk + d;
+        int tv = __builtin_amdgcn_readlane(o[0], ti & 63);
+#pragma unroll
+        for (int c = 1; c < NA; ++c) {
+            const int tc = __builtin_amdgcn_readlane(o[c], ti & 63);
+            tv = (ti >> 6) == c ? tc : tv;
+        }
+        const int want = d >= tka ? tlen + 1 : -7;
+        stop = tv == want ? -1 : stop;
+    } while (d < stop);
+    return stop < 0 ? d | REG_HIT : d;
 }
 
 // A pair's descriptor and, when both sequences total at most 64 words, its
@@ -631,21 +738,22 @@ __device__ __forceinline__ PairPrefetch prefetch_pair(const AlignArgs& a, uint32
     return f;
 }
 
-// Overlapped copies of both sequences from prefetched registers: lane j's
+// Little-endian copies of both sequences from prefetched registers: lane j's
 // next word arrives from lane j+1 by a DPP whole-wave left shift.
 __device__ __forceinline__ void stage_from_regs(uint64_t* P, uint64_t* T, const PairPrefetch& f, int lane) {
     const int nwp = ((int)f.pr.plen + 31) >> 5, nwt = ((int)f.pr.tlen + 31) >> 5;
     const uint32_t lo = (uint32_t)f.word, hi = (uint32_t)(f.word >> 32);
-    const uint32_t nlo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x130, 0xF, 0xF, false);
     const uint32_t nhi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x130, 0xF, 0xF, false);
     const bool is_p = lane < nwp;
     const int m = is_p ? lane : lane - nwp;
     const int nw = is_p ? nwp : nwt;
-    const uint64_t b = (m + 1 < nw) ? (((uint64_t)nhi << 32) | nlo) : 0ull;
+    const uint64_t gn = (m + 1 < nw) ? ((uint64_t)nhi << 32) : 0ull;
     uint64_t* S = is_p ? P : T;
     if (lane < nwp + nwt) {
-        S[2 * m] = f.word;
-        S[2 * m + 1] = (f.word << 32) | (b >> 32);
+        uint64_t w0, w1;
+        le_words(((uint64_t)hi << 32) | lo, gn, w0, w1);
+        S[2 * m] = w0;
+        S[2 * m + 1] = w1;
     }
     if (lane < 2) {
         P[2 * nwp + lane] = 0;
@@ -657,15 +765,15 @@ __device__ __forceinline__ void stage_from_regs(uint64_t* P, uint64_t* T, const 
 // live in NC VGPRs: chunk c, lane j holds diagonal k = 64c + j - d.  The next
 // level reads the same lane (k+1), the lane before (k) and the lane two
 // before (k-1): two DPP whole-wave right shifts, with the chunk below's
-// lanes 62/63 entering lane 0/1 through readlane.  Diagonals outside [-d, d]
-// hold -1.  Levels up to 32*NC-1 fit.
+// lanes 62/63 entering lane 0/1 through readlane.  Levels 32(n-1)..32n-1
+// take n chunks, each range in a loop of its own.  Levels up to 32*NC-1 fit.
 template <int NC>
 __device__ __forceinline__ void align_pair_reg(const AlignArgs& a, uint32_t pidx, const PairPrefetch& f, uint64_t* slot, int lane) {
     const ewfa_pair_t& pr = f.pr;
     const int plen = (int)pr.plen, tlen = (int)pr.tlen;
     const int sw = a.seq_words;
     const int nwp = (plen + 31) >> 5, nwt = (tlen + 31) >> 5;
-    // sw: words of one overlapped sequence copy
+    // sw: words of one sequence copy
     if (pr.plen > a.max_len || pr.tlen > a.max_len || 2 * nwp + 2 > sw || 2 * nwt + 2 > sw) {
         if (lane == 0) a.results[pidx] = ewfa_result_t{0, 0, EWFA_PAIR_TOO_LONG, 0};
         return;
@@ -676,20 +784,24 @@ __device__ __forceinline__ void align_pair_reg(const AlignArgs& a, uint32_t pidx
     if (f.in_regs) {
         stage_from_regs(P, T, f, lane);
     } else {
-        stage_overlapped(P, a.words + pr.pattern_off, plen, lane);
-        stage_overlapped(T, a.words + pr.text_off, tlen, lane);
+        stage_le(P, a.words + pr.pattern_off, plen, lane);
+        stage_le(T, a.words + pr.text_off, tlen, lane);
     }
     group_sync<1>();
 
     const int tk = tlen - plen;
-    const int tka = tk < 0 ? -tk : tk;
-    int o[NC];
+    int o[NC], nidx[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) o[c] = -1;
-    const int o00 = extend_o(P, T, plen, tlen, 0, 0);
-    if (lane == 0) o[0] = o00;
-    // run scratch of this wave (the emit is fused)
+    for (int c = 0; c < NC; ++c) {
+        o[c] = 0;
+        nidx[c] = -((c << 6) + lane);
+    }
+    const int o00 = extend_le(P, T, plen, tlen, 0, 0);
+    if (lane == 0) o[0] = o00 + 1;
+    // run scratch of this wave (the emit is fused), also as a buffer whose
+    // size bounds the level loops' stores to the slot
     uint8_t* R = a.runs + (size_t)blockIdx.x * a.runs_slot;
+    const __amdgpu_buffer_rsrc_t Rs = __builtin_amdgcn_make_buffer_rsrc(R, (short)0, (int)a.runs_slot, BUF_DWORD3);
     if (lane == 0) R[0] = (uint8_t)min(o00, 255);
 
     int limit = min(a.dcap, 32 * NC - 1);
@@ -697,23 +809,18 @@ __device__ __forceinline__ void align_pair_reg(const AlignArgs& a, uint32_t pidx
     if (cap > 0 && cap - 1 < limit) limit = cap - 1;
     bool done = (tk == 0) && (o00 == tlen);
     int d = 0;
-    while (!done && d < limit) {
-        ++d;
-        uint64_t* BTlev = BT + 2 * (size_t)bt_level_base((uint32_t)d);
-        uint8_t* Rlev = R + run_at(d, 0);
-        const int nch = (d >> 5) + 1;
-        if (nch == 1) reg_level<1, NC>(o, d, lane, P, T, plen, tlen, BTlev, Rlev);
-        else if (NC >= 2 && nch == 2) reg_level<(NC >= 2 ? 2 : 1), NC>(o, d, lane, P, T, plen, tlen, BTlev, Rlev);
-        else if (NC >= 3 && nch == 3) reg_level<(NC >= 3 ? 3 : 1), NC>(o, d, lane, P, T, plen, tlen, BTlev, Rlev);
-        else if (NC >= 4) reg_level<(NC >= 4 ? 4 : 1), NC>(o, d, lane, P, T, plen, tlen, BTlev, Rlev);
-        if (tka <= d) {
-            const int ti = tk + d;  // flattened index of the target diagonal
-            int tv = -1;
+    // one loop per chunk count; each starts where the last stopped
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
-                if ((ti >> 6) == c) tv = __builtin_amdgcn_readlane(o[c], ti & 63);
-            done = (tv == tlen);
-        }
+    for (int na = 1; na <= NC; ++na) {
+        const int dend = min(limit, 32 * na - 1);
+        if (done || d >= dend) continue;
+        int r;
+        if (na == 1) r = reg_levels<1, NC>(o, nidx, d, dend, lane, P, T, plen, tlen, tk, BT, Rs);
+        else if (na == 2) r = reg_levels<(NC >= 2 ? 2 : 1), NC>(o, nidx, d, dend, lane, P, T, plen, tlen, tk, BT, Rs);
+        else if (na == 3) r = reg_levels<(NC >= 3 ? 3 : 1), NC>(o, nidx, d, dend, lane, P, T, plen, tlen, tk, BT, Rs);
+        else r = reg_levels<(NC >= 4 ? 4 : 1), NC>(o, nidx, d, dend, lane, P, T, plen, tlen, tk, BT, Rs);
+        d = r & (REG_HIT - 1);
+        done = (r & REG_HIT) != 0;
     }
     group_sync<1>();
     finish_pair<true>(a, pidx, BT, lane, done, d, limit, tk, &pr, R);
