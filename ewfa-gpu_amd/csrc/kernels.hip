@@ -57,44 +57,8 @@ __device__ __forceinline__ int extend(Ptr pat, Ptr txt, int plen, int tlen, int 
     return acc;
 }
 
-// 32 bases from base i of an LDS sequence copy (words i>>5 and i>>5+1 must be
-// readable; the copy carries a zero pad word).  Branchless funnel shift: the
-// second word moves right by 64-s as (b >> 1) >> (63 - s), which is 0 for s = 0.
-__device__ __forceinline__ uint64_t window32(const uint64_t* S, int i) {
-    const int w = i >> 5;
-    const uint32_t s = (uint32_t)(i & 31) << 1;
-    const uint64_t a = S[w];
-    const uint64_t b = S[w + 1];
-    return (a << s) | ((b >> 1) >> (63u - s));
-}
-
-// One 32-base comparison step from pattern position v, text position h:
-// the number of equal bases before the first mismatch or either end (0..32).
-// live == false (no base left, or an inactive lane) gives 0 without reading
-// past the copies.
-__device__ __forceinline__ int step32(const uint64_t* P, const uint64_t* T, int plen, int tlen, int v, int h,
-                                      bool live) {
-    const int rem = min(plen - v, tlen - h);
-    live = live && rem > 0 && v >= 0 && h >= 0;
-    const uint64_t x0 = window32(P, live ? v : 0) ^ window32(T, live ? h : 0);
-    const int r = live ? min(rem, 32) : 0;
-    const uint64_t x = x0 | ((r >= 32) ? 0ull : (~0ull >> (2 * r)));
-    return x ? (int)(__clzll(x) >> 1) : 32;
-}
-
-// Full extension from (v, h) on LDS copies.
-__device__ __forceinline__ int extend_lds(const uint64_t* P, const uint64_t* T, int plen, int tlen, int v, int h) {
-    int acc = 0;
-    for (;;) {
-        const int e = step32(P, T, plen, tlen, v + acc, h + acc, true);
-        acc += e;
-        if (e < 32) break;
-    }
-    return acc;
-}
-
 // ---- LDS copies as 32-bit words ------------------------------------------
-// The align and emit kernels keep their LDS copies as 32-bit words in base
+// The LDS-staged emit kernel keeps its copies as 32-bit words in base
 // order (word q = bases 16q..16q+15, first base in the high bits), written
 // by swapping the halves of each 64-bit global word.  Two zero pad words
 // (four 32-bit words) follow the data.
@@ -137,7 +101,7 @@ __device__ __forceinline__ int extend_w(const uint32_t* P, const uint32_t* T, in
     return acc;
 }
 
-// ---- overlapped LDS copies (align kernel) ---------------------------------
+// ---- overlapped LDS copies (wave-per-pair emit) ---------------------------
 // Word q of an overlapped copy holds 32-bit words q and q+1 of the sequence
 // (bases 16q..16q+31, first base in the high bits).  From the global 64-bit
 // words G: word 2m = G[m], word 2m+1 = (G[m] << 32) | (G[m+1] >> 32).  A
@@ -152,26 +116,6 @@ __device__ __forceinline__ uint64_t win32o(const uint64_t* S, int i) {
     const uint64_t x1 = S[w] << s;
     const uint64_t x2 = S[w + 1] << s;
     return (x1 & 0xFFFFFFFF00000000ull) | (x2 >> 32);
-}
-
-// One 32-base step: matched bases from pattern position v and text position
-// h, capped at the bases left before either end (0 when inactive).
-__device__ __forceinline__ int step32o(const uint64_t* P, const uint64_t* T, int plen, int tlen, int v, int h,
-                                       bool live) {
-    const int rem = live ? max(min(plen - v, tlen - h), 0) : 0;
-    const int vc = min(max(v, 0), plen), hc = min(max(h, 0), tlen);
-    const int eq = lead_eq(win32o(P, vc) ^ win32o(T, hc));
-    return min(eq, rem);
-}
-
-__device__ __forceinline__ int extend_o(const uint64_t* P, const uint64_t* T, int plen, int tlen, int v, int h) {
-    int acc = 0;
-    for (;;) {
-        const int e = step32o(P, T, plen, tlen, v + acc, h + acc, true);
-        acc += e;
-        if (e < 32) break;
-    }
-    return acc;
 }
 
 // Stages one sequence of len bases from global words g into an overlapped
@@ -432,6 +376,39 @@ __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, c
     if (lane == 0) *res = ewfa_result_t{dist, 0, status, 0};
 }
 
+// Little-endian 32-bit copies for the block kernel: word j holds bases
+// 16j..16j+15, base 16j+r in bits 2r+1..2r with its two bits swapped (the
+// bit reversal of the packed big-endian 64-bit word is two such words, low
+// first).  Only base equality is asked, which the swap keeps.  Two zero
+// 64-bit words follow the data.
+
+// Equal leading bases (0..32) of the 32-base windows at pattern position v
+// and text position h: three words and two funnel shifts per sequence.
+__device__ __forceinline__ int eq_le3(const uint32_t* P, const uint32_t* T, int v, int h) {
+    const uint32_t* p = P + (v >> 4);
+    const uint32_t* t = T + (h >> 4);
+    const uint32_t sv = (uint32_t)v << 1, sh = (uint32_t)h << 1;
+    const uint32_t xl = __builtin_amdgcn_alignbit(p[1], p[0], sv) ^ __builtin_amdgcn_alignbit(t[1], t[0], sh);
+    const uint32_t xh = __builtin_amdgcn_alignbit(p[2], p[1], sv) ^ __builtin_amdgcn_alignbit(t[2], t[1], sh);
+    const uint64_t x = ((uint64_t)xh << 32) | xl;
+    const uint32_t z = x ? (uint32_t)__builtin_ctzll(x) : 64u;
+    return (int)(z >> 1);
+}
+
+// Equal bases from (v, h) up to the first mismatch or either end.  Positions
+// past an end extend by nothing; the reads are clamped into the copies.
+__device__ __forceinline__ int extend_le3(const uint32_t* P, const uint32_t* T, int plen, int tlen, int v, int h) {
+    int acc = max(min(eq_le3(P, T, min(v, plen), min(h, tlen)), min(plen - v, tlen - h)), 0);
+    if (acc == 32) {
+        int e;
+        do {
+            e = min(eq_le3(P, T, v + acc, h + acc), min(plen - v - acc, tlen - h - acc));
+            acc += e;
+        } while (e == 32);
+    }
+    return acc;
+}
+
 template <int G, bool BT_LDS>
 __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, uint64_t* bt_g,
                            int gtid) {
@@ -442,32 +419,31 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
     ewfa_result_t* res = a.results + pidx;
     const int sw = a.seq_words;
     const int nwp = (plen + 31) >> 5, nwt = (tlen + 31) >> 5;
-    if (pr.plen > a.max_len || pr.tlen > a.max_len || nwp + 1 > sw || nwt + 1 > sw) {
+    if (pr.plen > a.max_len || pr.tlen > a.max_len || nwp + 2 > sw || nwt + 2 > sw) {
         if (gtid == 0) *res = ewfa_result_t{0, 0, EWFA_PAIR_TOO_LONG, 0};
         return;
     }
     const int dcap = a.dcap;
     uint64_t* P = slot;
     uint64_t* T = slot + sw;
+    const uint32_t* P32 = reinterpret_cast<const uint32_t*>(P);
+    const uint32_t* T32 = reinterpret_cast<const uint32_t*>(T);
     int32_t* X = reinterpret_cast<int32_t*>(slot + 2 * sw) + (dcap + 2);
     int32_t* Y = X + (2 * dcap + 5);
     uint64_t* BT = BT_LDS ? slot + slot_bt_word(sw, dcap) : bt_g;
 
-    // stage the two sequences (coalesced 8-byte loads) plus a zero pad word
+    // stage the two sequences little-endian (coalesced 8-byte loads) plus
+    // two zero pad words
     const uint64_t* gp = a.words + pr.pattern_off;
     const uint64_t* gt = a.words + pr.text_off;
-    for (int i = gtid; i < nwp; i += 64 * G) P[i] = gp[i];
-    for (int i = gtid; i < nwt; i += 64 * G) T[i] = gt[i];
-    if (gtid == 0) {
-        P[nwp] = 0;
-        T[nwt] = 0;
-    }
+    for (int i = gtid; i < nwp + 2; i += 64 * G) P[i] = i < nwp ? __builtin_bitreverse64(gp[i]) : 0ull;
+    for (int i = gtid; i < nwt + 2; i += 64 * G) T[i] = i < nwt ? __builtin_bitreverse64(gt[i]) : 0ull;
     group_sync<G>();
 
     const int tk = tlen - plen;  // target diagonal, target offset tlen
     const int tka = tk < 0 ? -tk : tk;
     if (gtid == 0) {
-        X[0] = extend_lds(P, T, plen, tlen, 0, 0);
+        X[0] = extend_le3(P32, T32, plen, tlen, 0, 0);
         X[-1] = -1;
         X[1] = -1;
         X[-2] = -1;
@@ -489,28 +465,18 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
         for (int c = wave; c < nch; c += G) {
             const int idx = (c << 6) + lane;  // diagonal index 0..2d
             const int k = idx - d;
-            int op = 0;
-            if (idx <= 2 * d) {
-                // candidates from level d-1 (src/kernels.cu:131-138); the max
-                // of (value, op) with DEL(3) > SUB(2) > INS(1) on ties
-                // (src/kernels.cu:141-152)
-                const int del = cur[k + 1];
-                const int sub = cur[k] + 1;
-                const int ins = cur[k - 1] + 1;
-                int val = del;
-                op = OP_DEL;
-                if (sub > val) {
-                    val = sub;
-                    op = OP_SUB;
-                }
-                if (ins > val) {
-                    val = ins;
-                    op = OP_INS;
-                }
-                nxt[k] = val + extend_lds(P, T, plen, tlen, val - k, val);
+            const bool valid = idx <= 2 * d;
+            int m = 0;
+            if (valid) {
+                // candidates from level d-1 (src/kernels.cu:131-138) packed
+                // as 4*value + op, so one max gives the value and DEL(3) >
+                // SUB(2) > INS(1) on ties (src/kernels.cu:141-152)
+                m = max(max(cur[k + 1] * 4 + OP_DEL, cur[k] * 4 + (4 + OP_SUB)), cur[k - 1] * 4 + (4 + OP_INS));
+                const int val = m >> 2;
+                nxt[k] = val + extend_le3(P32, T32, plen, tlen, val - k, val);
             }
-            const uint64_t lo = __ballot(op & 1);
-            const uint64_t hi = __ballot(op & 2);
+            const uint64_t lo = __builtin_amdgcn_ballot_w64(valid && (m & 1) != 0);
+            const uint64_t hi = __builtin_amdgcn_ballot_w64(valid && (m & 2) != 0);
             if (lane == 0) {
                 uint64_t* q = BT + 2 * (size_t)(base + (uint32_t)c);
                 q[0] = lo;
