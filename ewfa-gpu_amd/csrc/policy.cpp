@@ -22,8 +22,8 @@ Policy default_policy() {
     // edits; longer pairs take a 16-wave block with as many levels as LDS holds
     //              len_hi   waves dcap  bt_lds grab
     p.buckets.push_back({300u, 1, 127, 1, 16});
-    p.buckets.push_back({700u, 1, 95, 1, 4});
-    p.buckets.push_back({1200u, 1, 127, 1, 1});
+    p.buckets.push_back({700u, 1, 127, 1, 4});
+    p.buckets.push_back({1200u, 1, 127, 1, 4});
     p.buckets.push_back({5000u, 2, 512, 0, 1});
     p.buckets.push_back({10500u, 4, 1024, 0, 1});
     p.buckets.push_back({0xFFFFFFFFu, 16, 1 << 20, 0, 1});
