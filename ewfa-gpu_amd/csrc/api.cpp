@@ -179,7 +179,7 @@ int max_dcap_for(int seq_words, int bt_lds, size_t budget) {
 }
 
 int finish_pass(ewfa_ctx* c, Pass& p, uint32_t items) {
-    p.lds = p.reg ? ewfa::lds_bytes_reg(p.seq_words, p.dcap) : ewfa::lds_bytes(p.seq_words, p.dcap, p.bt_lds);
+    p.lds = p.reg ? ewfa::lds_bytes_reg(p.seq_words, p.dcap, p.reg) : ewfa::lds_bytes(p.seq_words, p.dcap, p.bt_lds);
     if (p.lds > lds_budget(c))
         return fail(EWFA_ERR_RANGE, "pass needs %zu bytes of LDS (device has %zu)", p.lds, c->lds_max);
     if (p.lds > 65536) {
@@ -216,7 +216,7 @@ int pass_from_bucket(ewfa_ctx* c, const ewfa::BucketPolicy& b, uint32_t blen, ui
     if (p.waves == 1 && p.bt_lds) {
         const int nc = ewfa::reg_chunks_for(p.dcap);
         const int ow = overlapped_words_for(blen);
-        if (nc > 0 && ewfa::lds_bytes_reg(ow, p.dcap) <= 65536) {
+        if (nc > 0 && ewfa::lds_bytes_reg(ow, p.dcap, nc) <= 65536) {
             p.reg = nc;
             p.seq_words = ow;
         }
@@ -339,6 +339,7 @@ int reserve_scratch(ewfa_ctx* c, size_t n, uint32_t max_len) {
 
 // Run scratch of a register pass with the emit fused (bytes per wave).
 uint32_t runs_slot_of(const Pass& p) {
+    if (p.reg == 1) return ewfa::runs_slot_bytes_band((uint32_t)std::min(p.dcap, 63));
     return ewfa::runs_slot_bytes((uint32_t)std::min(p.dcap, 32 * p.reg - 1));
 }
 

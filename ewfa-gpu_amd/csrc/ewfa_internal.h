@@ -110,15 +110,23 @@ inline size_t lds_bytes(int seq_words, int dcap, int bt_lds) {
 }
 
 // LDS bytes of a register-wavefront slot (sequences + backtrace only).
-inline size_t lds_bytes_reg(int seq_words, int dcap) {
-    return 16u * (size_t)seq_words + 16u * (size_t)bt_chunks((uint32_t)dcap);
+// (one chunk: the banded wavefront, one ballot pair per level)
+inline size_t lds_bytes_reg(int seq_words, int dcap, int nc) {
+    const size_t bt = nc == 1 ? (size_t)dcap : (size_t)bt_chunks((uint32_t)dcap);
+    return 16u * (size_t)seq_words + 16u * bt;
 }
 
 // Register chunks the register wavefront needs for dcap levels (0: too many).
+// Up to 63 levels the banded wavefront holds them in one chunk.
 inline int reg_chunks_for(int dcap) {
+    if (dcap <= 63) return 1;
     const int nc = (2 * dcap + 1 + 63) / 64;
-    return nc <= 4 ? (nc < 1 ? 1 : nc) : 0;
+    return nc <= 4 ? nc : 0;
 }
+
+// Bytes of a wave's run scratch in the banded wavefront: a 64-byte row per
+// level 0..dmax.
+__host__ __device__ inline uint32_t runs_slot_bytes_band(uint32_t dmax) { return ((dmax + 1) * 64u + 127u) & ~127u; }
 
 // Bytes of a pair's CIGAR region the emit kernel wrote: whole dwords from
 // cigar_off up to and including the terminator (or up to cigar_cap when the

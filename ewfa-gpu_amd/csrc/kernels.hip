@@ -201,8 +201,18 @@ __device__ __forceinline__ uint32_t run_at(int level, int idx) { return (uint32_
 // the run scratch, and its byte offset by a prefix sum, so the stream and its
 // terminator come out as the emit kernel writes them (src/kernels.cu:398-498).
 // A path with a run of 255 or more bases is left to the emit kernel.
+// Run scratch byte of (level st, diagonal k).  Triangular layout: row st at
+// st*st, diagonal k at k+st.  Band layout (band >= 0): row st at 64*st,
+// diagonal k at lane k+band.  k is clamped into the level's diagonals, and
+// the index into the row, so a read never leaves the row.
+__device__ __forceinline__ uint32_t run_index(int st, int k, int band) {
+    const int kc = min(max(k, -st), st);
+    if (band < 0) return run_at(st, kc + st);
+    return (uint32_t)(64 * st + min(max(kc + band, 0), 63));
+}
+
 __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, const ewfa_pair_t& pr, const uint8_t* R,
-                                           int dist, int start, int status, int tk, int lane) {
+                                           int dist, int start, int status, int tk, int lane, int band) {
     ewfa_result_t* res = a.results + pidx;
     if ((pr.cigar_off & 3u) || (pr.cigar_cap & 3u)) {
         if (lane == 0) *res = ewfa_result_t{dist, 0, EWFA_PAIR_BADARG, 0};
@@ -233,7 +243,7 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
     bool big = false;
     int fin = 0;
     if (status != EWFA_PAIR_UNDEFINED && stop == nsteps) {
-        fin = R[run_at(dist, min(max(tk + dist, 0), 2 * dist))];
+        fin = R[run_index(dist, tk, band)];
         big = fin >= 255;
     }
     int run0 = 0, run1 = 0;
@@ -245,9 +255,9 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
         const int incl = r ? 0 : wave_incl_scan_i32(x);  // the whole wave takes part
         const int k = r ? tk + (tot1 - x1_incl + x1) : tk + tot1 + ((int)__builtin_amdgcn_readlane(incl, 63) - incl + x);
         if (st < nsteps && st <= stop) {
-            // the path's diagonal at level st lies in [-st, st]; the clamp
-            // only keeps the read inside the scratch row
-            const int run = R[run_at(st, min(max(k + st, 0), 2 * st))];
+            // the path's diagonal at level st lies in [-st, st] (and in the
+            // band); the clamp only keeps the read inside the scratch row
+            const int run = R[run_index(st, k, band)];
             big |= run >= 255;
             if (r) run1 = run; else run0 = run;
         }
@@ -287,7 +297,7 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
 template <bool FUSED>
 __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, const uint64_t* BT, int gtid,
                                             bool done, int d, int limit, int tk, const ewfa_pair_t* prp = nullptr,
-                                            const uint8_t* R = nullptr) {
+                                            const uint8_t* R = nullptr, int band = -1) {
     ewfa_result_t* res = a.results + pidx;
     const int tka = tk < 0 ? -tk : tk;
     const int cap = a.max_distance;
@@ -317,50 +327,100 @@ __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, c
         uint64_t word = 0;
         int k = tk;
         const int last = tka <= start ? 1 : start + 1;  // no walk when |t| > start
-        for (int top = start; top >= last; top -= 64) {
-            const int lev_l = top - lane;
-            const int c0 = max(0, (k + top - 2 * lane) >> 6);
-            uint64_t r[3][2];
-#pragma unroll
-            for (int u = 0; u < 3; ++u) r[u][0] = r[u][1] = 0;
-            if (lev_l >= 1) {
-                const uint64_t* row = BT + 2 * (size_t)bt_level_base((uint32_t)lev_l);
-                const int nch = (lev_l >> 5) + 1;
-#pragma unroll
-                for (int u = 0; u < 3; ++u)
-                    if (c0 + u < nch) {
-                        r[u][0] = row[2 * (c0 + u)];
-                        r[u][1] = row[2 * (c0 + u) + 1];
-                    }
-            }
-            const int steps = min(64, top - last + 1);
-            for (int j = 0; j < steps; ++j) {
-                const int lev = top - j;
-                const int s = lev - 1;
-                while (wi > (s >> 5)) {
-                    if (lane == 0) ow[wi] = word;
-                    --wi;
-                    word = 0;
+        // band layout: level l's two ballots at BT[2(l-1)], diagonal k at bit
+        // k+band
+        if (band >= 0) {
+            // steps s = start-1 down to last-1, op s from level s+1; each
+            // 32-bit half of an ops word (16 steps) fills from its top step
+            // down, shifting in two bits a step
+            uint32_t* ow32 = reinterpret_cast<uint32_t*>(ow);
+            uint32_t hw = 0;  // the half word being filled
+            for (int top = start; top >= last; top -= 64) {
+                const int lev_l = top - lane;
+                uint64_t rl = 0, rh = 0;
+                if (lev_l >= 1) {
+                    rl = BT[2 * (lev_l - 1)];
+                    rh = BT[2 * (lev_l - 1) + 1];
                 }
-                const int idx = k + lev;
-                const int rel = (idx >> 6) - __builtin_amdgcn_readlane(c0, j);
-                const int bit = idx & 63;
-                uint32_t wl = 0, wh = 0;
-#pragma unroll
-                for (int u = 0; u < 3; ++u)
-                    if (rel == u) {
+                const int s_end = max(top - 64, last - 1);  // lowest step of the window
+                int s = top - 1;
+                while (s >= s_end) {
+                    // the steps of this window inside half word s>>4
+                    const int w_end = max(s_end, s & ~15);
+                    for (; s >= w_end; --s) {
+                        const int j = top - 1 - s;
+                        const int bit = k + band;
+                        uint32_t wl, wh;
                         if (bit >= 32) {
-                            wl = __builtin_amdgcn_readlane((int)(uint32_t)(r[u][0] >> 32), j);
-                            wh = __builtin_amdgcn_readlane((int)(uint32_t)(r[u][1] >> 32), j);
+                            wl = __builtin_amdgcn_readlane((int)(uint32_t)(rl >> 32), j);
+                            wh = __builtin_amdgcn_readlane((int)(uint32_t)(rh >> 32), j);
                         } else {
-                            wl = __builtin_amdgcn_readlane((int)(uint32_t)r[u][0], j);
-                            wh = __builtin_amdgcn_readlane((int)(uint32_t)r[u][1], j);
+                            wl = __builtin_amdgcn_readlane((int)(uint32_t)rl, j);
+                            wh = __builtin_amdgcn_readlane((int)(uint32_t)rh, j);
                         }
+                        const int b = bit & 31;
+                        const int op = (int)((wl >> b) & 1u) + 2 * (int)((wh >> b) & 1u);
+                        hw = (hw << 2) | (uint32_t)op;
+                        k += op - 2;
                     }
-                const int b = bit & 31;
-                const int op = (int)((wl >> b) & 1u) | (int)(((wh >> b) & 1u) << 1);
-                word |= (uint64_t)op << ((s & 31) << 1);
-                k += op - 2;
+                    if ((s & 15) == 15) {  // step s+1 opened its half word: complete
+                        if (lane == 0) ow32[(s + 1) >> 4] = hw;
+                        hw = 0;
+                    }
+                }
+            }
+            // the walk is all steps down to 0 (last == 1) or none; the half
+            // words above the walked ones, up to step dist-1, hold zero ops
+            const int walked_top = (last == 1 && start >= 1) ? ((start - 1) >> 4) : -1;
+            for (int q = 2 * wi + 1; q > walked_top; --q)
+                if (lane == 0) ow32[q] = 0;
+            wi = -1;
+        } else {
+            for (int top = start; top >= last; top -= 64) {
+                const int lev_l = top - lane;
+                const int c0 = max(0, (k + top - 2 * lane) >> 6);
+                uint64_t r[3][2];
+#pragma unroll
+                for (int u = 0; u < 3; ++u) r[u][0] = r[u][1] = 0;
+                if (lev_l >= 1) {
+                    const uint64_t* row = BT + 2 * (size_t)bt_level_base((uint32_t)lev_l);
+                    const int nch = (lev_l >> 5) + 1;
+#pragma unroll
+                    for (int u = 0; u < 3; ++u)
+                        if (c0 + u < nch) {
+                            r[u][0] = row[2 * (c0 + u)];
+                            r[u][1] = row[2 * (c0 + u) + 1];
+                        }
+                }
+                const int steps = min(64, top - last + 1);
+                for (int j = 0; j < steps; ++j) {
+                    const int lev = top - j;
+                    const int s = lev - 1;
+                    while (wi > (s >> 5)) {
+                        if (lane == 0) ow[wi] = word;
+                        --wi;
+                        word = 0;
+                    }
+                    const int idx = k + lev;
+                    const int rel = (idx >> 6) - __builtin_amdgcn_readlane(c0, j);
+                    const int bit = idx & 63;
+                    uint32_t wl = 0, wh = 0;
+#pragma unroll
+                    for (int u = 0; u < 3; ++u)
+                        if (rel == u) {
+                            if (bit >= 32) {
+                                wl = __builtin_amdgcn_readlane((int)(uint32_t)(r[u][0] >> 32), j);
+                                wh = __builtin_amdgcn_readlane((int)(uint32_t)(r[u][1] >> 32), j);
+                            } else {
+                                wl = __builtin_amdgcn_readlane((int)(uint32_t)r[u][0], j);
+                                wh = __builtin_amdgcn_readlane((int)(uint32_t)r[u][1], j);
+                            }
+                        }
+                    const int b = bit & 31;
+                    const int op = (int)((wl >> b) & 1u) | (int)(((wh >> b) & 1u) << 1);
+                    word |= (uint64_t)op << ((s & 31) << 1);
+                    k += op - 2;
+                }
             }
         }
         while (wi >= 0) {
@@ -370,7 +430,7 @@ __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, c
         }
     }
     if constexpr (FUSED) {
-        fused_emit(a, pidx, *prp, R, dist, start, status, tk, lane);
+        fused_emit(a, pidx, *prp, R, dist, start, status, tk, lane, band);
         return;
     }
     if (lane == 0) *res = ewfa_result_t{dist, 0, status, 0};
@@ -733,8 +793,124 @@ __device__ __forceinline__ void stage_from_regs(uint64_t* P, uint64_t* T, const 
 // before (k-1): two DPP whole-wave right shifts, with the chunk below's
 // lanes 62/63 entering lane 0/1 through readlane.  Levels 32(n-1)..32n-1
 // take n chunks, each range in a loop of its own.  Levels up to 32*NC-1 fit.
+// Banded register wavefront: one VGPR of 64 diagonals for passes that search
+// at most L <= 63 levels.  Only the band of cells that can still reach the
+// target diagonal tk within L levels matters: (d, k) with |k - tk| <= L - d.
+// A cell (d', k') that feeds a cell (d, k) of the band has |k' - k| <= d - d',
+// so |k' - tk| <= L - d': the band is closed under taking sources, and the
+// traced path, its runs and the target diagonal's offsets all lie in it.
+// Over all levels the band spans the L+1 diagonals -floor((L-tk)/2) ..
+// ceil((L+tk)/2), so lane j holds diagonal j - A with A = floor((L - tk)/2)
+// throughout: the previous level's k+1, k and k-1 sit in lanes j+1, j and
+// j-1.  The lanes of diagonals -d..d are computed as in the full wavefront;
+// a lane next to the window's edge misses a source there, but that error
+// moves inward one lane per level and never reaches the band (its front at
+// level d is below lane d-A-1 <= lowest band lane, and likewise at the top
+// edge), so for a pair that reaches its target within L levels the distance,
+// ops and runs along the path are those of the full wavefront.  A pair that
+// does not is deferred (or capped) exactly as before.  Level d's ballots go
+// to BT[2(d-1)], its runs to row 64*d of the run scratch, lane by lane.
+__device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pidx, const PairPrefetch& f,
+                                                uint64_t* slot, int lane) {
+    const ewfa_pair_t& pr = f.pr;
+    const int plen = (int)pr.plen, tlen = (int)pr.tlen;
+    const int sw = a.seq_words;
+    const int nwp = (plen + 31) >> 5, nwt = (tlen + 31) >> 5;
+    if (pr.plen > a.max_len || pr.tlen > a.max_len || 2 * nwp + 2 > sw || 2 * nwt + 2 > sw) {
+        if (lane == 0) a.results[pidx] = ewfa_result_t{0, 0, EWFA_PAIR_TOO_LONG, 0};
+        return;
+    }
+    uint64_t* P = slot;
+    uint64_t* T = slot + sw;
+    uint64_t* BT = slot + 2 * sw;
+    if (f.in_regs) {
+        stage_from_regs(P, T, f, lane);
+    } else {
+        stage_le(P, a.words + pr.pattern_off, plen, lane);
+        stage_le(T, a.words + pr.text_off, tlen, lane);
+    }
+    group_sync<1>();
+
+    const int tk = tlen - plen;
+    const int tka = tk < 0 ? -tk : tk;
+    int limit = min(a.dcap, 63);
+    const int cap = a.max_distance;
+    if (cap > 0 && cap - 1 < limit) limit = cap - 1;
+    const int A = (limit - tk) >> 1;  // lane of diagonal 0 (when |tk| <= limit)
+    const int o00 = extend_le(P, T, plen, tlen, 0, 0);
+    // run scratch of this wave (the emit is fused), also as a buffer whose
+    // size bounds the level loop's stores to the slot
+    uint8_t* R = a.runs + (size_t)blockIdx.x * a.runs_slot;
+    const __amdgpu_buffer_rsrc_t Rs = __builtin_amdgcn_make_buffer_rsrc(R, (short)0, (int)a.runs_slot, BUF_DWORD3);
+    if (lane == 0) R[min(max(A, 0), 63)] = (uint8_t)min(o00, 255);
+    bool done = (tk == 0) && (o00 == tlen);
+    int d = 0;
+    if (!done && tka <= limit && limit >= 1) {
+        int o = lane == A ? o00 + 1 : 0;  // offset + 1; 0: none
+        const int ca = A - lane;           // v = h + ca on lane's diagonal
+        const int tl = A + tk;             // lane of the target diagonal
+        uint64_t* BTl = BT;
+        // lanes of the wavefront's diagonals -d..d, clipped to the window
+        uint64_t vm = 1ull << A;
+        int stop = limit;  // one exit: drops to -1 when the target is reached
+        do {
+            ++d;
+            vm |= (vm << 1) | (vm >> 1);
+            const bool valid = __builtin_amdgcn_inverse_ballot_w64(vm);
+            // candidates (src/kernels.cu:131-138): DEL from k+1 (lane j+1),
+            // SUB from k (lane j), INS from k-1 (lane j-1), packed as
+            // 4*offset + op so one max gives the offset and DEL(3) > SUB(2) >
+            // INS(1) on ties (src/kernels.cu:141-152)
+            const int sdel = __builtin_amdgcn_update_dpp(0, o, 0x130, 0xF, 0xF, true);
+            const int sins = __builtin_amdgcn_update_dpp(0, o, 0x138, 0xF, 0xF, true);
+            const int m = max(max(sdel * 4 + (OP_DEL - 4), o * 4 + OP_SUB), sins * 4 + OP_INS);
+            const int h = m >> 2;
+            int acc = 0;
+            if (valid) {
+                const int v = h + ca;
+                acc = max(min(eq_le(P, T, v, h), min(plen - v, tlen - h)), 0);
+            }
+            if (__builtin_amdgcn_ballot_w64(acc == 32)) {
+                int last = acc;
+                int hh = h + acc, vv = hh + ca;
+                do {
+                    if (last == 32) {
+                        last = min(eq_le(P, T, vv, hh), min(plen - vv, tlen - hh));
+                        hh += last;
+                        vv += last;
+                    }
+                } while (__builtin_amdgcn_ballot_w64(last == 32));
+                acc = hh - h;
+            }
+            __builtin_amdgcn_raw_buffer_store_b8((uint8_t)min(acc, 255), Rs, lane, 64 * d, 0);
+            o = valid ? h + 1 + acc : 0;
+            const uint64_t bl = __builtin_amdgcn_ballot_w64((m & 1) != 0);
+            const uint64_t bh = __builtin_amdgcn_ballot_w64((m & 2) != 0);
+            if (lane == 0) {
+                BTl[0] = bl;
+                BTl[1] = bh;
+            }
+            BTl += 2;
+            // the target diagonal's offset; before level |tk| it is outside
+            // the wavefront, and no offset equals -7
+            const int tv = __builtin_amdgcn_readlane(o, tl);
+            const int want = d >= tka ? tlen + 1 : -7;
+            stop = tv == want ? -1 : stop;
+        } while (d < stop);
+        done = stop < 0;
+    }
+    group_sync<1>();
+    // (when |tk| > limit nothing is walked, and row 0 is read where it was
+    // written)
+    finish_pair<true>(a, pidx, BT, lane, done, d, limit, tk, &pr, R, min(max(A, 0), 63));
+}
+
 template <int NC>
 __device__ __forceinline__ void align_pair_reg(const AlignArgs& a, uint32_t pidx, const PairPrefetch& f, uint64_t* slot, int lane) {
+    if constexpr (NC == 1) {
+        align_pair_band(a, pidx, f, slot, lane);
+        return;
+    }
     const ewfa_pair_t& pr = f.pr;
     const int plen = (int)pr.plen, tlen = (int)pr.tlen;
     const int sw = a.seq_words;
