@@ -327,8 +327,8 @@ __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, c
         uint64_t word = 0;
         int k = tk;
         const int last = tka <= start ? 1 : start + 1;  // no walk when |t| > start
-        // band layout: level l's two ballots at BT[2(l-1)], diagonal k at bit
-        // k+band
+        // band layout: level l's two ballots at BT[2(l-1)] (DEL won, SUB
+        // won), diagonal k at bit k+band
         if (band >= 0) {
             // steps s = start-1 down to last-1, op s from level s+1; each
             // 32-bit half of an ops word (16 steps) fills from its top step
@@ -359,7 +359,8 @@ __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, c
                             wh = __builtin_amdgcn_readlane((int)(uint32_t)rh, j);
                         }
                         const int b = bit & 31;
-                        const int op = (int)((wl >> b) & 1u) + 2 * (int)((wh >> b) & 1u);
+                        // (DEL won, SUB won) -> DEL 3, SUB 2, else INS 1
+                        const int op = ((wl >> b) & 1u) ? OP_DEL : ((wh >> b) & 1u) ? OP_SUB : OP_INS;
                         hw = (hw << 2) | (uint32_t)op;
                         k += op - 2;
                     }
@@ -595,6 +596,13 @@ __device__ __forceinline__ int eq_le(const uint64_t* P, const uint64_t* T, int v
     const uint64_t x = ((uint64_t)xh << 32) | xl;
     const uint32_t z = x ? (uint32_t)__builtin_ctzll(x) : 64u;
     return (int)(z >> 1);
+}
+
+// Median of a, b and c in one instruction.
+__device__ __forceinline__ int med3_i32(int a, int b, int c) {
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
 }
 
 // Little-endian copy of a len-base sequence from global words g, cooperatively.
@@ -848,6 +856,7 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
     if (!done && tka <= limit && limit >= 1) {
         int o = lane == A ? o00 + 1 : 0;  // offset + 1; 0: none
         const int ca = A - lane;           // v = h + ca on lane's diagonal
+        const int mc = min(plen - ca, tlen);  // bases left from offset h: mc - h
         const int tl = A + tk;             // lane of the target diagonal
         uint64_t* BTl = BT;
         // lanes of the wavefront's diagonals -d..d, clipped to the window
@@ -863,19 +872,20 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
             // INS(1) on ties (src/kernels.cu:141-152)
             const int sdel = __builtin_amdgcn_update_dpp(0, o, 0x130, 0xF, 0xF, true);
             const int sins = __builtin_amdgcn_update_dpp(0, o, 0x138, 0xF, 0xF, true);
-            const int m = max(max(sdel * 4 + (OP_DEL - 4), o * 4 + OP_SUB), sins * 4 + OP_INS);
+            const int cdel = sdel * 4 + (OP_DEL - 4), csub = o * 4 + OP_SUB;
+            const int m = max(max(cdel, csub), sins * 4 + OP_INS);
             const int h = m >> 2;
             int acc = 0;
             if (valid) {
-                const int v = h + ca;
-                acc = max(min(eq_le(P, T, v, h), min(plen - v, tlen - h)), 0);
+                // matched bases, capped at the bases left (none past an end)
+                acc = med3_i32(eq_le(P, T, h + ca, h), mc - h, 0);
             }
             if (__builtin_amdgcn_ballot_w64(acc == 32)) {
                 int last = acc;
                 int hh = h + acc, vv = hh + ca;
                 do {
                     if (last == 32) {
-                        last = min(eq_le(P, T, vv, hh), min(plen - vv, tlen - hh));
+                        last = min(eq_le(P, T, vv, hh), mc - hh);
                         hh += last;
                         vv += last;
                     }
@@ -884,8 +894,10 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
             }
             __builtin_amdgcn_raw_buffer_store_b8((uint8_t)min(acc, 255), Rs, lane, 64 * d, 0);
             o = valid ? h + 1 + acc : 0;
-            const uint64_t bl = __builtin_amdgcn_ballot_w64((m & 1) != 0);
-            const uint64_t bh = __builtin_amdgcn_ballot_w64((m & 2) != 0);
+            // op ballots: DEL won, SUB won (else INS); each candidate has its
+            // own op in its low bits, so exactly one equals the max
+            const uint64_t bl = __builtin_amdgcn_ballot_w64(m == cdel);
+            const uint64_t bh = __builtin_amdgcn_ballot_w64(m == csub);
             if (lane == 0) {
                 BTl[0] = bl;
                 BTl[1] = bh;
