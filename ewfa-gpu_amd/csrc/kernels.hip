@@ -212,7 +212,8 @@ __device__ __forceinline__ uint32_t run_index(int st, int k, int band) {
 }
 
 __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, const ewfa_pair_t& pr, const uint8_t* R,
-                                           int dist, int start, int status, int tk, int lane, int band) {
+                                           int dist, int start, int status, int tk, int lane, int band,
+                                           bool longrun) {
     ewfa_result_t* res = a.results + pidx;
     if ((pr.cigar_off & 3u) || (pr.cigar_cap & 3u)) {
         if (lane == 0) *res = ewfa_result_t{dist, 0, EWFA_PAIR_BADARG, 0};
@@ -239,12 +240,14 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
         if (z0) stop = __ffsll((long long)z0) - 1;
         else if (z1) stop = 64 + __ffsll((long long)z1) - 1;
     }
-    // pass 1: runs and byte counts; pass 2 (no run of 255 or more): the bytes
-    bool big = false;
+    // pass 1: runs and byte counts; pass 2 (no run of 255 or more): the bytes.
+    // longrun: the scratch may hold a run's low byte only, so the pair goes to
+    // the emit kernel
+    bool big = longrun;
     int fin = 0;
     if (status != EWFA_PAIR_UNDEFINED && stop == nsteps) {
         fin = R[run_index(dist, tk, band)];
-        big = fin >= 255;
+        big = big || fin >= 255;
     }
     int run0 = 0, run1 = 0;
 #pragma unroll
@@ -297,7 +300,7 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
 template <bool FUSED>
 __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, const uint64_t* BT, int gtid,
                                             bool done, int d, int limit, int tk, const ewfa_pair_t* prp = nullptr,
-                                            const uint8_t* R = nullptr, int band = -1) {
+                                            const uint8_t* R = nullptr, int band = -1, bool longrun = false) {
     ewfa_result_t* res = a.results + pidx;
     const int tka = tk < 0 ? -tk : tk;
     const int cap = a.max_distance;
@@ -431,7 +434,7 @@ __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, c
         }
     }
     if constexpr (FUSED) {
-        fused_emit(a, pidx, *prp, R, dist, start, status, tk, lane, band);
+        fused_emit(a, pidx, *prp, R, dist, start, status, tk, lane, band, longrun);
         return;
     }
     if (lane == 0) *res = ewfa_result_t{dist, 0, status, 0};
@@ -853,6 +856,7 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
     if (lane == 0) R[min(max(A, 0), 63)] = (uint8_t)min(o00, 255);
     bool done = (tk == 0) && (o00 == tlen);
     int d = 0;
+    bool longrun = false;  // some level ran a continuation of 7+ steps
     if (!done && tka <= limit && limit >= 1) {
         int o = lane == A ? o00 + 1 : 0;  // offset + 1; 0: none
         const int ca = A - lane;           // v = h + ca on lane's diagonal
@@ -870,11 +874,14 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
             // SUB from k (lane j), INS from k-1 (lane j-1), packed as
             // 4*offset + op so one max gives the offset and DEL(3) > SUB(2) >
             // INS(1) on ties (src/kernels.cu:141-152)
+            // in stored units the candidates are DEL o(k+1), SUB o(k)+1 and
+            // INS o(k-1)+1; DEL wins ties, then SUB
             const int sdel = __builtin_amdgcn_update_dpp(0, o, 0x130, 0xF, 0xF, true);
             const int sins = __builtin_amdgcn_update_dpp(0, o, 0x138, 0xF, 0xF, true);
-            const int cdel = sdel * 4 + (OP_DEL - 4), csub = o * 4 + OP_SUB;
-            const int m = max(max(cdel, csub), sins * 4 + OP_INS);
-            const int h = m >> 2;
+            const int si = max(o, sins) + 1;
+            const int st = max(sdel, si);  // the new offset + 1
+            const bool is_del = sdel >= si, is_sub = o >= sins;
+            const int h = st - 1;
             int acc = 0;
             if (valid) {
                 // matched bases, capped at the bases left (none past an end)
@@ -883,21 +890,27 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
             if (__builtin_amdgcn_ballot_w64(acc == 32)) {
                 int last = acc;
                 int hh = h + acc, vv = hh + ca;
+                int it = 0;
                 do {
                     if (last == 32) {
                         last = min(eq_le(P, T, vv, hh), mc - hh);
                         hh += last;
                         vv += last;
                     }
+                    ++it;
                 } while (__builtin_amdgcn_ballot_w64(last == 32));
                 acc = hh - h;
+                // a run of 255 bases or more takes 32 + 7 * 32 + more: seven
+                // continuation steps at least
+                longrun |= it >= 7;
             }
-            __builtin_amdgcn_raw_buffer_store_b8((uint8_t)min(acc, 255), Rs, lane, 64 * d, 0);
-            o = valid ? h + 1 + acc : 0;
-            // op ballots: DEL won, SUB won (else INS); each candidate has its
-            // own op in its low bits, so exactly one equals the max
-            const uint64_t bl = __builtin_amdgcn_ballot_w64(m == cdel);
-            const uint64_t bh = __builtin_amdgcn_ballot_w64(m == csub);
+            // the run's low byte (a pair with a run of 255+ is emitted by the
+            // emit kernel, which recomputes the runs)
+            __builtin_amdgcn_raw_buffer_store_b8((uint8_t)acc, Rs, lane, 64 * d, 0);
+            o = valid ? st + acc : 0;
+            // op ballots: DEL won, SUB won (else INS)
+            const uint64_t bl = __builtin_amdgcn_ballot_w64(is_del);
+            const uint64_t bh = __builtin_amdgcn_ballot_w64(is_sub) & ~bl;
             if (lane == 0) {
                 BTl[0] = bl;
                 BTl[1] = bh;
@@ -914,7 +927,7 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
     group_sync<1>();
     // (when |tk| > limit nothing is walked, and row 0 is read where it was
     // written)
-    finish_pair<true>(a, pidx, BT, lane, done, d, limit, tk, &pr, R, min(max(A, 0), 63));
+    finish_pair<true>(a, pidx, BT, lane, done, d, limit, tk, &pr, R, min(max(A, 0), 63), longrun);
 }
 
 template <int NC>
