@@ -120,3 +120,29 @@ def test_band_under_reference_cap(tmp_path, cap):
     check(res, pairs, cig, P, T, f"cap{cap}", max_distance=cap)
     st = res["status"]
     assert ((st == ewfa.PAIR_CAPPED) | (st == ewfa.PAIR_UNDEFINED)).any()
+
+
+def test_band_long_runs(tmp_path):
+    """Runs at the byte limits: the band kernel keeps only a run's low byte in
+    its scratch and sends a pair with a run of 255 or more to the emit
+    kernel; streams must equal the oracle's on both sides of the limit."""
+    rng = random.Random(77)
+    P, T = [], []
+    for run in (200, 254, 255, 256, 257, 300, 510, 511, 512, 513, 767, 768, 1000):
+        for edits in (1, 3):
+            head = rand_seq(rng, 20)
+            body = rand_seq(rng, run)
+            p = head + body
+            t = bytearray(p)
+            for pos in rng.sample(range(len(head)), edits):
+                t[pos] = ord("A") if t[pos] != ord("A") else ord("C")
+            P.append(p)
+            T.append(bytes(t))
+            P.append(body + head)
+            T.append(body + rand_seq(rng, 5) + head)
+    say(f"case band long runs {len(P)} pairs")
+    words, pairs, cb, ml = ewfa.from_sequences(P, T)
+    path = policy_file(tmp_path, "band63", "4000000000 1 63 1 4\n")
+    with ewfa.Context(policy_path=path) as ctx:
+        res, cig = ctx.align_host(words, pairs, len(P), cb)
+    check(res, pairs, cig, P, T, "longruns")
