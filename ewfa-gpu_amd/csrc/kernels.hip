@@ -18,6 +18,19 @@
 
 namespace ewfa {
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Word 3 of a raw byte buffer resource on gfx9 parts (32-bit data format, so
+// untyped accesses are in range; bounds checked against the record count).
+constexpr int BUF_DWORD3 = 0x00020000;
+
+// Median of a, b and c in one instruction.
+__device__ __forceinline__ int med3_i32(int a, int b, int c) {
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // sequence access
 
@@ -421,7 +434,8 @@ __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, c
                             }
                         }
                     const int b = bit & 31;
-                    const int op = (int)((wl >> b) & 1u) | (int)(((wh >> b) & 1u) << 1);
+                    // (DEL won, SUB won) -> DEL 3, SUB 2, else INS 1
+                    const int op = ((wl >> b) & 1u) ? OP_DEL : ((wh >> b) & 1u) ? OP_SUB : OP_INS;
                     word |= (uint64_t)op << ((s & 31) << 1);
                     k += op - 2;
                 }
@@ -477,7 +491,7 @@ template <int G, bool BT_LDS>
 __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, uint64_t* bt_g,
                            int gtid) {
     const int lane = gtid & 63;
-    const int wave = gtid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(gtid >> 6);  // uniform in the wave
     const ewfa_pair_t pr = a.pairs[pidx];
     const int plen = (int)pr.plen, tlen = (int)pr.tlen;
     ewfa_result_t* res = a.results + pidx;
@@ -495,6 +509,9 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
     int32_t* X = reinterpret_cast<int32_t*>(slot + 2 * sw) + (dcap + 2);
     int32_t* Y = X + (2 * dcap + 5);
     uint64_t* BT = BT_LDS ? slot + slot_bt_word(sw, dcap) : bt_g;
+    // the HBM backtrace slot as a buffer: stores take their offset in a scalar
+    const __amdgpu_buffer_rsrc_t BTs =
+        __builtin_amdgcn_make_buffer_rsrc(bt_g, (short)0, (int)(a.bt_slot_words * 8u), BUF_DWORD3);
 
     // stage the two sequences little-endian (coalesced 8-byte loads) plus
     // two zero pad words
@@ -530,21 +547,44 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
             const int idx = (c << 6) + lane;  // diagonal index 0..2d
             const int k = idx - d;
             const bool valid = idx <= 2 * d;
-            int m = 0;
+            uint64_t lo = 0, hi = 0;  // op ballots of the chunk's valid lanes
             if (valid) {
                 // candidates from level d-1 (src/kernels.cu:131-138) packed
                 // as 4*value + op, so one max gives the value and DEL(3) >
                 // SUB(2) > INS(1) on ties (src/kernels.cu:141-152)
-                m = max(max(cur[k + 1] * 4 + OP_DEL, cur[k] * 4 + (4 + OP_SUB)), cur[k - 1] * 4 + (4 + OP_INS));
+                const int32_t* q = cur + (k - 1);
+                const int cdel = q[2] * 4 + OP_DEL;
+                const int csub = q[1] * 4 + (4 + OP_SUB);
+                const int m = max(max(cdel, csub), q[0] * 4 + (4 + OP_INS));
+                // DEL won, SUB won (else INS): each candidate carries its own
+                // op in its low bits, so exactly one equals the max
+                lo = __builtin_amdgcn_ballot_w64(m == cdel);
+                hi = __builtin_amdgcn_ballot_w64(m == csub);
                 const int val = m >> 2;
-                nxt[k] = val + extend_le3(P32, T32, plen, tlen, val - k, val);
+                const int v = val - k;
+                // matched bases, capped at the bases left (none past an end);
+                // a valid offset is never left of either start, and a read
+                // past an end stays inside the slot
+                const int mc = min(plen + k, tlen);
+                int acc = med3_i32(eq_le3(P32, T32, v, val), mc - val, 0);
+                if (acc == 32) {
+                    int e;
+                    do {
+                        e = min(eq_le3(P32, T32, v + acc, val + acc), mc - val - acc);
+                        acc += e;
+                    } while (e == 32);
+                }
+                nxt[k] = val + acc;
             }
-            const uint64_t lo = __builtin_amdgcn_ballot_w64(valid && (m & 1) != 0);
-            const uint64_t hi = __builtin_amdgcn_ballot_w64(valid && (m & 2) != 0);
             if (lane == 0) {
-                uint64_t* q = BT + 2 * (size_t)(base + (uint32_t)c);
-                q[0] = lo;
-                q[1] = hi;
+                if constexpr (BT_LDS) {
+                    uint64_t* q = BT + 2 * (size_t)(base + (uint32_t)c);
+                    q[0] = lo;
+                    q[1] = hi;
+                } else {
+                    const u32x4 w = {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
+                    __builtin_amdgcn_raw_buffer_store_b128(w, BTs, 0, (int)(16u * (base + (uint32_t)c)), 0);
+                }
             }
         }
         // sentinels for the next level's reads at |k| = d+1, d+2
@@ -563,9 +603,6 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
 }
 
 // ---- register wavefront (one wave per pair) --------------------------------
-// Word 3 of a raw byte buffer resource on gfx9 parts (32-bit data format, so
-// untyped accesses are in range; bounds checked against the record count).
-constexpr int BUF_DWORD3 = 0x00020000;
 
 // Its LDS copies are little-endian: 32-bit word j holds bases 16j..16j+15,
 // base 16j+r in bits 2r+1..2r with its two bits swapped (the bit reversal of
@@ -599,13 +636,6 @@ __device__ __forceinline__ int eq_le(const uint64_t* P, const uint64_t* T, int v
     const uint64_t x = ((uint64_t)xh << 32) | xl;
     const uint32_t z = x ? (uint32_t)__builtin_ctzll(x) : 64u;
     return (int)(z >> 1);
-}
-
-// Median of a, b and c in one instruction.
-__device__ __forceinline__ int med3_i32(int a, int b, int c) {
-    int r;
-    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
 }
 
 // Little-endian copy of a len-base sequence from global words g, cooperatively.
@@ -650,7 +680,7 @@ __device__ __forceinline__ void reg_level(int (&o)[NC], const int (&nidx)[NC], i
         p62[c] = __builtin_amdgcn_readlane(o[c], 62);
         p63[c] = __builtin_amdgcn_readlane(o[c], 63);
     }
-    int val[NA], acc[NA], m[NA];
+    int val[NA], acc[NA], m[NA], cdel[NA], csub[NA];
     bool valid[NA];
 #pragma unroll
     for (int c = 0; c < NA; ++c) {
@@ -668,7 +698,9 @@ __device__ __forceinline__ void reg_level(int (&o)[NC], const int (&nidx)[NC], i
         // through unchanged
         // (offsets: DEL o-1, SUB s1, INS s2 in stored units, so the max is
         // the new offset itself)
-        m[c] = max(max(o[c] * 4 + (OP_DEL - 4), s1 * 4 + OP_SUB), s2 * 4 + OP_INS);
+        cdel[c] = o[c] * 4 + (OP_DEL - 4);
+        csub[c] = s1 * 4 + OP_SUB;
+        m[c] = max(max(cdel[c], csub[c]), s2 * 4 + OP_INS);
         // lanes 0..2d-64c hold the chunk's diagonals: all of a chunk below
         // the top one, vtop of the top one
         valid[c] = c + 1 < NA ? true : __builtin_amdgcn_inverse_ballot_w64(vtop);
@@ -708,8 +740,10 @@ __device__ __forceinline__ void reg_level(int (&o)[NC], const int (&nidx)[NC], i
 #pragma unroll
     for (int c = 0; c < NA; ++c) {
         const uint64_t vm = c + 1 < NA ? ~0ull : vtop;
-        const uint64_t lo = __builtin_amdgcn_ballot_w64((m[c] & 1) != 0) & vm;
-        const uint64_t hi = __builtin_amdgcn_ballot_w64((m[c] & 2) != 0) & vm;
+        // DEL won, SUB won (else INS): each candidate carries its own op in
+        // its low bits, so exactly one equals the max
+        const uint64_t lo = __builtin_amdgcn_ballot_w64(m[c] == cdel[c]) & vm;
+        const uint64_t hi = __builtin_amdgcn_ballot_w64(m[c] == csub[c]) & vm;
         if (lane == 0) {
             BTl[2 * c] = lo;
             BTl[2 * c + 1] = hi;
