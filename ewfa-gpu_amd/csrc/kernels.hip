@@ -543,6 +543,7 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
         ++d;
         const int nch = (d >> 5) + 1;
         const uint32_t base = bt_level_base((uint32_t)d);
+        const int32_t* curm1 = cur - 1;
         for (int c = wave; c < nch; c += G) {
             const int idx = (c << 6) + lane;  // diagonal index 0..2d
             const int k = idx - d;
@@ -552,7 +553,7 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
                 // candidates from level d-1 (src/kernels.cu:131-138) packed
                 // as 4*value + op, so one max gives the value and DEL(3) >
                 // SUB(2) > INS(1) on ties (src/kernels.cu:141-152)
-                const int32_t* q = cur + (k - 1);
+                const int32_t* q = curm1 + k;  // diagonals k-1, k, k+1
                 const int cdel = q[2] * 4 + OP_DEL;
                 const int csub = q[1] * 4 + (4 + OP_SUB);
                 const int m = max(max(cdel, csub), q[0] * 4 + (4 + OP_INS));
