@@ -1,15 +1,24 @@
 // kernels.hip -- gfx950 kernels of the WFA edit aligner.
 //
+//   align_reg_kernel<NC>     one wave per pair, the wavefront in registers:
+//                            NC = 1 is the banded wavefront (one 64-diagonal
+//                            window placed on the target diagonal, passes of
+//                            up to 63 levels), NC = 2..4 the full wavefront in
+//                            NC chunks.  Ops as two 64-bit ballots per level
+//                            and chunk in LDS, traceback, and the CIGAR stream
+//                            written by the same wave (fused emit).
 //   align_kernel<G, BT_LDS>  wavefront search for one pair per group of G
 //                            wave64s: lanes = diagonals, offsets staged in LDS,
-//                            2-bit ops per cell recorded as two 64-bit ballots
-//                            per (level, 64 diagonals), traceback at the end.
-//                            Replaces WF_edit_distance + WF_compute_kernel +
-//                            WF_extend_kernel (src/kernels.cu:27-310).
-//   emit_kernel              one lane per pair: forward walk over the ops with
-//                            re-extension, writes the compressed CIGAR stream.
-//                            Replaces generate_cigars[_sh_mem]
-//                            (src/kernels.cu:398-625).
+//                            ops as two 64-bit ballots per (level, 64
+//                            diagonals) in LDS or spilled to HBM, traceback at
+//                            the end.
+//                            Together these replace WF_edit_distance +
+//                            WF_compute_kernel + WF_extend_kernel
+//                            (src/kernels.cu:27-310).
+//   emit_kernel (and the LDS-staged and wave-per-pair variants)
+//                            forward walk over the ops with re-extension,
+//                            writes the compressed CIGAR stream.  Replaces
+//                            generate_cigars[_sh_mem] (src/kernels.cu:398-625).
 //   pack_kernel              ASCII -> 2-bit words.  Replaces compact_sequences
 //                            (src/kernels.cu:313-394).
 //
@@ -547,21 +556,17 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
         for (int c = wave; c < nch; c += G) {
             const int idx = (c << 6) + lane;  // diagonal index 0..2d
             const int k = idx - d;
-            const bool valid = idx <= 2 * d;
-            uint64_t lo = 0, hi = 0;  // op ballots of the chunk's valid lanes
-            if (valid) {
-                // candidates from level d-1 (src/kernels.cu:131-138) packed
-                // as 4*value + op, so one max gives the value and DEL(3) >
-                // SUB(2) > INS(1) on ties (src/kernels.cu:141-152)
+            // lane 0 of every chunk up to nch-1 is inside the level
+            if (idx <= 2 * d) {
+                // candidates from level d-1 (src/kernels.cu:131-138): DEL
+                // o(k+1), SUB o(k)+1, INS o(k-1)+1; the larger wins, DEL on
+                // ties, then SUB (src/kernels.cu:141-152)
                 const int32_t* q = curm1 + k;  // diagonals k-1, k, k+1
-                const int cdel = q[2] * 4 + OP_DEL;
-                const int csub = q[1] * 4 + (4 + OP_SUB);
-                const int m = max(max(cdel, csub), q[0] * 4 + (4 + OP_INS));
-                // DEL won, SUB won (else INS): each candidate carries its own
-                // op in its low bits, so exactly one equals the max
-                lo = __builtin_amdgcn_ballot_w64(m == cdel);
-                hi = __builtin_amdgcn_ballot_w64(m == csub);
-                const int val = m >> 2;
+                const int ins = q[0], sub = q[1], del = q[2];
+                const int si = max(sub, ins) + 1;
+                const int val = max(del, si);
+                const uint64_t lo = __builtin_amdgcn_ballot_w64(del >= si);  // DEL won
+                const uint64_t hi = __builtin_amdgcn_ballot_w64(sub >= ins) & ~lo;  // SUB won
                 const int v = val - k;
                 // matched bases, capped at the bases left (none past an end);
                 // a valid offset is never left of either start, and a read
@@ -576,15 +581,15 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
                     } while (e == 32);
                 }
                 nxt[k] = val + acc;
-            }
-            if (lane == 0) {
-                if constexpr (BT_LDS) {
-                    uint64_t* q = BT + 2 * (size_t)(base + (uint32_t)c);
-                    q[0] = lo;
-                    q[1] = hi;
-                } else {
-                    const u32x4 w = {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
-                    __builtin_amdgcn_raw_buffer_store_b128(w, BTs, 0, (int)(16u * (base + (uint32_t)c)), 0);
+                if (lane == 0) {
+                    if constexpr (BT_LDS) {
+                        uint64_t* bq = BT + 2 * (size_t)(base + (uint32_t)c);
+                        bq[0] = lo;
+                        bq[1] = hi;
+                    } else {
+                        const u32x4 w = {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
+                        __builtin_amdgcn_raw_buffer_store_b128(w, BTs, 0, (int)(16u * (base + (uint32_t)c)), 0);
+                    }
                 }
             }
         }
