@@ -686,8 +686,9 @@ __device__ __forceinline__ void reg_level(int (&o)[NC], const int (&nidx)[NC], i
         p62[c] = __builtin_amdgcn_readlane(o[c], 62);
         p63[c] = __builtin_amdgcn_readlane(o[c], 63);
     }
-    int val[NA], acc[NA], m[NA], cdel[NA], csub[NA];
+    int val[NA], acc[NA];
     bool valid[NA];
+    uint64_t bl[NA], bh[NA];  // op ballots: DEL won, SUB won (else INS)
 #pragma unroll
     for (int c = 0; c < NA; ++c) {
         int s1, s2;
@@ -698,26 +699,25 @@ __device__ __forceinline__ void reg_level(int (&o)[NC], const int (&nidx)[NC], i
             s1 = __builtin_amdgcn_update_dpp(p63[c - 1], o[c], 0x138, 0xF, 0xF, false);
             s2 = __builtin_amdgcn_update_dpp(p62[c - 1], s1, 0x138, 0xF, 0xF, false);
         }
-        // candidates (src/kernels.cu:131-138) packed as 4*value + op, so one
-        // max gives the value and DEL(3) > SUB(2) > INS(1) on ties
-        // (src/kernels.cu:141-152); the +1 of the stored offsets carries
-        // through unchanged
-        // (offsets: DEL o-1, SUB s1, INS s2 in stored units, so the max is
-        // the new offset itself)
-        cdel[c] = o[c] * 4 + (OP_DEL - 4);
-        csub[c] = s1 * 4 + OP_SUB;
-        m[c] = max(max(cdel[c], csub[c]), s2 * 4 + OP_INS);
+        // candidates (src/kernels.cu:131-138) in stored units: DEL o (this
+        // lane held k+1), SUB s1+1, INS s2+1; the larger wins, DEL on ties,
+        // then SUB (src/kernels.cu:141-152)
+        const int si = max(s1, s2) + 1;
+        const int st = max(o[c], si);
         // lanes 0..2d-64c hold the chunk's diagonals: all of a chunk below
         // the top one, vtop of the top one
+        const uint64_t vm = c + 1 < NA ? ~0ull : vtop;
+        bl[c] = __builtin_amdgcn_ballot_w64(o[c] >= si) & vm;
+        bh[c] = __builtin_amdgcn_ballot_w64(s1 >= s2) & vm & ~bl[c];
         valid[c] = c + 1 < NA ? true : __builtin_amdgcn_inverse_ballot_w64(vtop);
-        val[c] = m[c] >> 2;
+        val[c] = st - 1;
         acc[c] = 0;
         // a valid offset is never left of either sequence start, and at most
         // d past an end: its reads stay inside the slot, and a position past
         // an end extends by nothing
         if (valid[c]) {
             const int h = val[c], v = h + nidx[c] + d;
-            acc[c] = max(min(eq_le(P, T, v, h), min(plen - v, tlen - h)), 0);
+            acc[c] = med3_i32(eq_le(P, T, v, h), min(plen - v, tlen - h), 0);
         }
     }
     // lanes whose first step matched all 32 bases continue; their positions
@@ -745,14 +745,9 @@ __device__ __forceinline__ void reg_level(int (&o)[NC], const int (&nidx)[NC], i
     }
 #pragma unroll
     for (int c = 0; c < NA; ++c) {
-        const uint64_t vm = c + 1 < NA ? ~0ull : vtop;
-        // DEL won, SUB won (else INS): each candidate carries its own op in
-        // its low bits, so exactly one equals the max
-        const uint64_t lo = __builtin_amdgcn_ballot_w64(m[c] == cdel[c]) & vm;
-        const uint64_t hi = __builtin_amdgcn_ballot_w64(m[c] == csub[c]) & vm;
         if (lane == 0) {
-            BTl[2 * c] = lo;
-            BTl[2 * c + 1] = hi;
+            BTl[2 * c] = bl[c];
+            BTl[2 * c + 1] = bh[c];
         }
     }
 }
