@@ -634,11 +634,12 @@ int resolve_deferred(ewfa_ctx* c, int prev_dcap, const uint64_t* words, const ew
 
 // ---------------------------------------------------------------------------
 // Host pipeline of the persistent path.  The pairs go in chunks (at most
-// PIPE_CHUNK pairs and PIPE_CAPS bytes of CIGAR regions each).  A chunk's
-// packed words and descriptors go up on the upload stream while the chunk
-// before it computes.  Each chunk's written CIGAR bytes are packed by the
-// device straight into pinned host memory, beside its results, and the host
-// places them while the next chunk computes.  Two stages alternate.
+// PIPE_CHUNK pairs and PIPE_CAPS bytes of CIGAR regions each).  All chunks'
+// packed words and descriptors are queued on the upload stream at once, so
+// chunk k computes as soon as its own uploads land.  Each chunk's written
+// CIGAR bytes are packed by the device straight into pinned host memory,
+// beside its results, and the host places them while the next chunk
+// computes.  Two stages alternate.
 
 constexpr size_t PIPE_CHUNK_MIN = (size_t)1 << 17;
 constexpr size_t PIPE_CHUNKS = 4;
@@ -677,20 +678,30 @@ int pipeline_setup(ewfa_ctx* c, size_t chunks) {
     return EWFA_OK;
 }
 
-// Enqueues chunk k: its uploads (words only when h_words is given), then on
-// the compute stream the persistent plan, the compaction into the stage's
-// pinned buffer and the results copy.
-int pipeline_enqueue(ewfa_ctx* c, const Chunk& ch, size_t k, const uint64_t* h_words, const ewfa_pair_t* h_pairs) {
+// Enqueues chunk k's uploads (words only when h_words is given) on the
+// upload stream.  Every chunk's words and descriptors have their own place
+// on the device, so all uploads go in at once.
+int pipeline_upload(ewfa_ctx* c, const Chunk& ch, size_t k, const uint64_t* h_words, const ewfa_pair_t* h_pairs) {
     uint64_t* d_words = c->h_words.as<uint64_t>();
     ewfa_pair_t* d_pairs = c->h_pairs.as<ewfa_pair_t>();
-    ewfa_result_t* d_results = c->h_results.as<ewfa_result_t>();
-    int8_t* d_cigars = c->h_cigars.as<int8_t>();
     const size_t n = ch.hi - ch.lo;
     if (h_words && ch.w_hi > ch.w_lo)
         HIPCHK(hipMemcpyAsync(d_words + ch.w_lo, h_words + ch.w_lo, (ch.w_hi - ch.w_lo) * 8, hipMemcpyHostToDevice,
                               c->up));
     HIPCHK(hipMemcpyAsync(d_pairs + ch.lo, h_pairs + ch.lo, n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, c->up));
     HIPCHK(hipEventRecord(c->ev_up[k], c->up));
+    return EWFA_OK;
+}
+
+// Enqueues chunk k on the compute stream, after its uploads: the persistent
+// plan, the compaction into the stage's pinned buffer and the results copy.
+// The stage must be free (its previous chunk placed by the host).
+int pipeline_enqueue(ewfa_ctx* c, const Chunk& ch, size_t k) {
+    uint64_t* d_words = c->h_words.as<uint64_t>();
+    ewfa_pair_t* d_pairs = c->h_pairs.as<ewfa_pair_t>();
+    ewfa_result_t* d_results = c->h_results.as<ewfa_result_t>();
+    int8_t* d_cigars = c->h_cigars.as<int8_t>();
+    const size_t n = ch.hi - ch.lo;
     HIPCHK(hipStreamWaitEvent(c->stream, c->ev_up[k], 0));
     ewfa_ctx::Stage& st = c->stage[k & 1];
     if (ch.caps > st.dense_cap || !st.dense) {
@@ -722,9 +733,8 @@ int pipeline_complete(ewfa_ctx* c, const Chunk& ch, size_t k, const ewfa_pair_t*
     HIPCHK(hipEventSynchronize(st.done));
     const size_t n = ch.hi - ch.lo;
     memcpy(h_results + ch.lo, st.res, n * sizeof(ewfa_result_t));
-    if (ewfa::compact_total(h_pairs + ch.lo, h_results + ch.lo, n) > ch.caps)
+    if (ewfa::scatter_streams(st.dense, ch.caps, h_pairs + ch.lo, h_results + ch.lo, n, h_cigars, 16) == SIZE_MAX)
         return fail(EWFA_ERR_STATE, "compacted streams exceed their regions");
-    ewfa::scatter_streams(st.dense, h_pairs + ch.lo, h_results + ch.lo, n, h_cigars, 8);
     return EWFA_OK;
 }
 
@@ -732,35 +742,6 @@ int pipeline_complete(ewfa_ctx* c, const Chunk& ch, size_t k, const ewfa_pair_t*
 // descriptors are checked while the chunks are formed.
 int run_pipeline(ewfa_ctx* c, const uint64_t* h_words, size_t n_words, const ewfa_pair_t* h_pairs, size_t n,
                  ewfa_result_t* h_results, int8_t* h_cigars, size_t cigar_bytes) {
-    std::vector<Chunk> chunks;
-    uint32_t max_len = 0;
-    size_t most = 0;
-    const size_t per = std::max(PIPE_CHUNK_MIN, (n + PIPE_CHUNKS - 1) / PIPE_CHUNKS);
-    for (size_t i = 0; i < n;) {
-        Chunk ch{i, i, SIZE_MAX, 0, 0, 0};
-        while (ch.hi < n && ch.hi - ch.lo < per &&
-               (ch.hi == ch.lo || ch.caps + h_pairs[ch.hi].cigar_cap <= PIPE_CAPS)) {
-            const ewfa_pair_t& p = h_pairs[ch.hi];
-            if (p.pattern_off + ewfa_packed_words(p.plen) - 1 > n_words ||
-                p.text_off + ewfa_packed_words(p.tlen) - 1 > n_words)
-                return fail(EWFA_ERR_ARG, "pair %zu reads past the packed words", ch.hi);
-            if ((p.cigar_off & 3) || (p.cigar_cap & 3) || p.cigar_off + p.cigar_cap > cigar_bytes)
-                return fail(EWFA_ERR_ARG, "pair %zu has a bad CIGAR region", ch.hi);
-            ch.w_lo = std::min<size_t>(ch.w_lo, std::min(p.pattern_off, p.text_off));
-            ch.w_hi = std::max<size_t>(ch.w_hi, std::max(p.pattern_off + ewfa_packed_words(p.plen),
-                                                         p.text_off + ewfa_packed_words(p.tlen)));
-            ch.caps += p.cigar_cap;
-            ch.max_len = std::max(ch.max_len, std::max(p.plen, p.tlen));
-            ++ch.hi;
-        }
-        ch.w_hi = std::min(ch.w_hi, n_words);
-        if (ch.w_lo > ch.w_hi) ch.w_lo = ch.w_hi;
-        max_len = std::max(max_len, ch.max_len);
-        most = std::max(most, ch.hi - ch.lo);
-        chunks.push_back(ch);
-        i = ch.hi;
-    }
-    CHK(pipeline_setup(c, chunks.size()));
     if (h_words) {
         CHK(c->h_words.reserve(std::max<size_t>(n_words, 1) * 8));
         c->resident_words = 0;  // only the words these pairs read go up
@@ -768,16 +749,53 @@ int run_pipeline(ewfa_ctx* c, const uint64_t* h_words, size_t n_words, const ewf
     CHK(c->h_pairs.reserve(n * sizeof(ewfa_pair_t)));
     CHK(c->h_results.reserve(n * sizeof(ewfa_result_t)));
     CHK(c->h_cigars.reserve(std::max<size_t>(cigar_bytes, 4)));
-    // size the per-chunk scratch for the largest chunk up front, so that no
-    // buffer is replaced while chunks are in flight
-    CHK(reserve_scratch(c, most, max_len));
     // earlier work on the context's stream finishes before the uploads land
     HIPCHK(hipEventRecord(c->ev_in, c->stream));
     HIPCHK(hipStreamWaitEvent(c->up, c->ev_in, 0));
+    // form the chunks, checking the descriptors; each chunk's uploads go in
+    // as soon as it is formed, so the checks of the next overlap them
+    std::vector<Chunk> chunks;
+    uint32_t max_len = 0;
+    size_t most = 0;
+    const size_t per = std::max(PIPE_CHUNK_MIN, (n + PIPE_CHUNKS - 1) / PIPE_CHUNKS);
     int rc = EWFA_OK;
+    for (size_t i = 0; i < n && rc == EWFA_OK;) {
+        Chunk ch{i, i, SIZE_MAX, 0, 0, 0};
+        while (ch.hi < n && ch.hi - ch.lo < per &&
+               (ch.hi == ch.lo || ch.caps + h_pairs[ch.hi].cigar_cap <= PIPE_CAPS)) {
+            const ewfa_pair_t& p = h_pairs[ch.hi];
+            if (p.pattern_off + ewfa_packed_words(p.plen) - 1 > n_words ||
+                p.text_off + ewfa_packed_words(p.tlen) - 1 > n_words) {
+                rc = fail(EWFA_ERR_ARG, "pair %zu reads past the packed words", ch.hi);
+                break;
+            }
+            if ((p.cigar_off & 3) || (p.cigar_cap & 3) || p.cigar_off + p.cigar_cap > cigar_bytes) {
+                rc = fail(EWFA_ERR_ARG, "pair %zu has a bad CIGAR region", ch.hi);
+                break;
+            }
+            ch.w_lo = std::min<size_t>(ch.w_lo, std::min(p.pattern_off, p.text_off));
+            ch.w_hi = std::max<size_t>(ch.w_hi, std::max(p.pattern_off + ewfa_packed_words(p.plen),
+                                                         p.text_off + ewfa_packed_words(p.tlen)));
+            ch.caps += p.cigar_cap;
+            ch.max_len = std::max(ch.max_len, std::max(p.plen, p.tlen));
+            ++ch.hi;
+        }
+        if (rc != EWFA_OK) break;
+        ch.w_hi = std::min(ch.w_hi, n_words);
+        if (ch.w_lo > ch.w_hi) ch.w_lo = ch.w_hi;
+        max_len = std::max(max_len, ch.max_len);
+        most = std::max(most, ch.hi - ch.lo);
+        chunks.push_back(ch);
+        rc = pipeline_setup(c, chunks.size());
+        if (rc == EWFA_OK) rc = pipeline_upload(c, ch, chunks.size() - 1, h_words, h_pairs);
+        i = ch.hi;
+    }
+    // size the per-chunk scratch for the largest chunk before any chunk
+    // computes, so that no buffer is replaced while chunks are in flight
+    if (rc == EWFA_OK) rc = reserve_scratch(c, most, max_len);
     size_t enq = 0, done = 0;
     for (size_t k = 0; k < chunks.size() && rc == EWFA_OK; ++k) {
-        rc = pipeline_enqueue(c, chunks[k], k, h_words, h_pairs);
+        rc = pipeline_enqueue(c, chunks[k], k);
         if (rc == EWFA_OK) ++enq;
         if (rc == EWFA_OK && k >= 1) {
             rc = pipeline_complete(c, chunks[k - 1], k - 1, h_pairs, h_results, h_cigars);
@@ -1232,8 +1250,10 @@ int ewfa_align_resident(ewfa_ctx_t* c, const ewfa_pair_t* h_pairs, size_t n, ewf
     HIPCHK(hipMemcpyAsync(h_results, c->h_results.p, n * sizeof(ewfa_result_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     const int8_t* dense = nullptr;
-    CHK(ewfa::fetch_compact(c, ewfa::compact_total(h_pairs, h_results, n), s, &dense));
-    ewfa::scatter_streams(dense, h_pairs, h_results, n, h_cigars, 8);
+    const size_t total = ewfa::compact_total(h_pairs, h_results, n);
+    CHK(ewfa::fetch_compact(c, total, s, &dense));
+    if (ewfa::scatter_streams(dense, total, h_pairs, h_results, n, h_cigars, 16) == SIZE_MAX)
+        return fail(EWFA_ERR_STATE, "compacted streams exceed their regions");
     return EWFA_OK;
 }
 

@@ -42,15 +42,33 @@ size_t compact_total(const ewfa_pair_t* pairs, const ewfa_result_t* results, siz
     return t;
 }
 
-void scatter_streams(const int8_t* dense, const ewfa_pair_t* pairs, const ewfa_result_t* results, size_t n,
-                     int8_t* h_cigars, int threads) {
-    std::vector<size_t> off(n + 1);
-    off[0] = 0;
-    for (size_t i = 0; i < n; ++i) off[i + 1] = off[i] + written_cigar_bytes(results[i], pairs[i]);
-    parallel_for(n, n >= 65536 ? threads : 1, [&](size_t lo, size_t hi) {
-        for (size_t i = lo; i < hi; ++i)
-            if (off[i + 1] > off[i]) memcpy(h_cigars + pairs[i].cigar_off, dense + off[i], off[i + 1] - off[i]);
+size_t scatter_streams(const int8_t* dense, size_t dense_cap, const ewfa_pair_t* pairs, const ewfa_result_t* results,
+                       size_t n, int8_t* h_cigars, int threads) {
+    // pass 1: each block's byte count; pass 2: each block places its pairs'
+    // streams from its own start in the dense buffer
+    const size_t t = n >= 65536 ? std::max(1, threads) : 1;
+    std::vector<size_t> start(t + 1, 0);
+    const size_t per = (n + t - 1) / t;
+    parallel_for(t, (int)t, [&](size_t lo, size_t hi) {
+        for (size_t b = lo; b < hi; ++b) {
+            size_t sum = 0;
+            for (size_t i = b * per; i < std::min(n, (b + 1) * per); ++i) sum += written_cigar_bytes(results[i], pairs[i]);
+            start[b + 1] = sum;
+        }
     });
+    for (size_t b = 0; b < t; ++b) start[b + 1] += start[b];
+    if (start[t] > dense_cap) return SIZE_MAX;
+    parallel_for(t, (int)t, [&](size_t lo, size_t hi) {
+        for (size_t b = lo; b < hi; ++b) {
+            size_t off = start[b];
+            for (size_t i = b * per; i < std::min(n, (b + 1) * per); ++i) {
+                const size_t w = written_cigar_bytes(results[i], pairs[i]);
+                if (w) memcpy(h_cigars + pairs[i].cigar_off, dense + off, w);
+                off += w;
+            }
+        }
+    });
+    return start[t];
 }
 
 }  // namespace ewfa

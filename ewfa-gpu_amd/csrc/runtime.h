@@ -43,8 +43,9 @@ int fetch_compact(ewfa_ctx_t* ctx, size_t total, hipStream_t stream, const int8_
 // Dense bytes of n pairs (host copies of descriptors and results).
 size_t compact_total(const ewfa_pair_t* pairs, const ewfa_result_t* results, size_t n);
 
-// Places each pair's dense bytes at its cigar_off in h_cigars.
-void scatter_streams(const int8_t* dense, const ewfa_pair_t* pairs, const ewfa_result_t* results, size_t n,
-                     int8_t* h_cigars, int threads);
+// Places each pair's dense bytes at its cigar_off in h_cigars and returns
+// their total, or SIZE_MAX (placing nothing) when that exceeds dense_cap.
+size_t scatter_streams(const int8_t* dense, size_t dense_cap, const ewfa_pair_t* pairs, const ewfa_result_t* results,
+                       size_t n, int8_t* h_cigars, int threads);
 
 }  // namespace ewfa

@@ -308,11 +308,13 @@ int deliver_batch(Flight* f) {
     if (b.n == 0) return EWFA_OK;
     if (ewfa::compact_total(f->pairs.p, f->results.p, b.n) > b.caps) return EWFA_ERR_STATE;
     const int8_t* dense = f->dense.p;
-    size_t k = 0;
+    size_t left = b.caps, k = 0;
     for (Job* j : b.jobs) {
         memcpy(j->results, f->results.p + k, j->n * sizeof(ewfa_result_t));
-        ewfa::scatter_streams(dense, j->pairs, j->results, j->n, j->cigars, 1);
-        dense += ewfa::compact_total(j->pairs, j->results, j->n);
+        const size_t w = ewfa::scatter_streams(dense, left, j->pairs, j->results, j->n, j->cigars, 1);
+        if (w == SIZE_MAX) return EWFA_ERR_STATE;
+        dense += w;
+        left -= w;
         k += j->n;
     }
     return EWFA_OK;
