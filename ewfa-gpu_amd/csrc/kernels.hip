@@ -43,41 +43,8 @@ __device__ __forceinline__ int med3_i32(int a, int b, int c) {
 // ---------------------------------------------------------------------------
 // sequence access
 
-// 32 bases starting at base i of a packed sequence (2 bits per base, first
-// base in the high bits).  LDS copies carry a zero pad word, so the second
-// word is always readable there; in global memory the read of the next word
-// is skipped when it lies past the sequence's data words.
-template <bool GUARD, typename Ptr>
-__device__ __forceinline__ uint64_t fetch32(Ptr seq, int i, int len) {
-    const int w = i >> 5;
-    const int s = (i & 31) << 1;
-    const uint64_t a = seq[w];
-    uint64_t b = 0;
-    if (!GUARD || (s != 0 && ((w + 1) << 5) < len)) b = seq[w + 1];
-    return s ? ((a << s) | (b >> (64 - s))) : a;
-}
-
-// Number of equal bases from pattern position v and text position h, stopping
-// at either end (the extend of src/kernels.cu:27-109, 32 bases per XOR+clz).
-template <bool GUARD, typename Ptr>
-__device__ __forceinline__ int extend(Ptr pat, Ptr txt, int plen, int tlen, int v, int h) {
-    int acc = 0;
-    for (;;) {
-        const int rem = min(plen - v, tlen - h);
-        if (rem <= 0) break;
-        uint64_t x = fetch32<GUARD>(pat, v, plen) ^ fetch32<GUARD>(txt, h, tlen);
-        if (rem < 32) x |= (~0ull) >> (rem << 1);  // bases past an end never match
-        if (x == 0) {
-            acc += 32;
-            v += 32;
-            h += 32;
-            continue;
-        }
-        acc += __clzll(x) >> 1;
-        break;
-    }
-    return acc;
-}
+// Sequences are packed 2 bits per base in 64-bit words, first base in the
+// high bits.
 
 // ---- LDS copies as 32-bit words ------------------------------------------
 // The LDS-staged emit kernel keeps its copies as 32-bit words in base
@@ -1144,6 +1111,49 @@ __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a) {
         emit_one(a, a.list ? a.list[item] : item);
 }
 
+// A packed sequence in global memory read through a two-word window: the
+// emit walk only moves forward, so each word is loaded about once.  window(i)
+// is called for 0 <= i < len only; the word after base i's is read only
+// while it holds data.
+struct GlobalSeq {
+    const uint64_t* g;
+    int len, w;
+    uint64_t a, b;
+    __device__ __forceinline__ GlobalSeq(const uint64_t* g_, int len_) : g(g_), len(len_), w(-2), a(0), b(0) {}
+    // 32 bases from base i (first base in the high bits)
+    __device__ __forceinline__ uint64_t window(int i) {
+        const int wi = i >> 5;
+        if (wi != w) {
+            a = wi == w + 1 ? b : g[wi];
+            b = ((wi + 1) << 5) < len ? g[wi + 1] : 0ull;
+            w = wi;
+        }
+        const int s = (i & 31) << 1;
+        return s ? ((a << s) | (b >> (64 - s))) : a;
+    }
+};
+
+// Equal bases from pattern position v, text position h, stopping at either
+// end (the extend of src/kernels.cu:27-109, 32 bases per XOR+clz).
+__device__ __forceinline__ int extend_seq(GlobalSeq& P, GlobalSeq& T, int v, int h) {
+    int acc = 0;
+    for (;;) {
+        const int rem = min(P.len - v, T.len - h);
+        if (rem <= 0) break;
+        uint64_t x = P.window(v) ^ T.window(h);
+        if (rem < 32) x |= (~0ull) >> (rem << 1);  // bases past an end never match
+        if (x == 0) {
+            acc += 32;
+            v += 32;
+            h += 32;
+            continue;
+        }
+        acc += __clzll(x) >> 1;
+        break;
+    }
+    return acc;
+}
+
 __device__ __forceinline__ void emit_one(const EmitArgs& a, uint32_t i) {
     ewfa_result_t r = a.results[i];
     if (r.status != EWFA_PAIR_OK && r.status != EWFA_PAIR_CAPPED && r.status != EWFA_PAIR_UNDEFINED)
@@ -1155,15 +1165,14 @@ __device__ __forceinline__ void emit_one(const EmitArgs& a, uint32_t i) {
     }
     StreamWriter w{a.cigars + pr.cigar_off, pr.cigar_cap, 0u, 0u};
     if (r.status != EWFA_PAIR_UNDEFINED) {
-        const int plen = (int)pr.plen, tlen = (int)pr.tlen;
-        const uint64_t* P = a.words + pr.pattern_off;
-        const uint64_t* T = a.words + pr.text_off;
+        GlobalSeq P(a.words + pr.pattern_off, (int)pr.plen);
+        GlobalSeq T(a.words + pr.text_off, (int)pr.tlen);
         const uint64_t* ow = a.ops + (size_t)i * a.ops_stride;
         int k = 0, off = 0;
         uint64_t word = 0;
         bool stopped = false;
         for (int s = 0; s < r.distance; ++s) {
-            const int e = extend<true>(P, T, plen, tlen, off - k, off);
+            const int e = extend_seq(P, T, off - k, off);
             w.run(e);
             off += e;
             if ((s & 31) == 0) word = ow[s >> 5];
@@ -1183,7 +1192,7 @@ __device__ __forceinline__ void emit_one(const EmitArgs& a, uint32_t i) {
                 break;
             }
         }
-        if (!stopped) w.run(extend<true>(P, T, plen, tlen, off - k, off));
+        if (!stopped) w.run(extend_seq(P, T, off - k, off));
     }
     w.finish();
     a.results[i].cigar_len = (int32_t)w.len;
