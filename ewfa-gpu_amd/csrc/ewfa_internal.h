@@ -109,11 +109,18 @@ inline size_t lds_bytes(int seq_words, int dcap, int bt_lds) {
     return b;
 }
 
-// LDS bytes of a register-wavefront slot (sequences + backtrace only).
-// (one chunk: the banded wavefront, one ballot pair per level)
+// Guard words of a banded slot before the pattern copy and after the text
+// copy: reads of lanes outside the wavefront, and of offsets past an end,
+// land in them.
+constexpr int BAND_HEAD_WORDS = 4;
+constexpr int BAND_TAIL_WORDS = 8;
+
+// LDS bytes of a register-wavefront slot (sequences + backtrace only).  One
+// chunk (the banded wavefront) keeps its backtrace in registers: the slot is
+// the two copies between the guards.
 inline size_t lds_bytes_reg(int seq_words, int dcap, int nc) {
-    const size_t bt = nc == 1 ? (size_t)dcap : (size_t)bt_chunks((uint32_t)dcap);
-    return 16u * (size_t)seq_words + 16u * bt;
+    if (nc == 1) return 8u * (size_t)(2 * seq_words + BAND_HEAD_WORDS + BAND_TAIL_WORDS);
+    return 16u * (size_t)seq_words + 16u * (size_t)bt_chunks((uint32_t)dcap);
 }
 
 // Register chunks the register wavefront needs for dcap levels (0: too many).

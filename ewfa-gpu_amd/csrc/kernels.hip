@@ -33,6 +33,14 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // untyped accesses are in range; bounds checked against the record count).
 constexpr int BUF_DWORD3 = 0x00020000;
 
+// x with lane `lane` set to the uniform value v (v_writelane, whatever the
+// exec mask).  The lane select goes through M0: one instruction reads one
+// scalar register besides M0.
+__device__ __forceinline__ int writelane_i32(int x, int v, int lane) {
+    asm("v_writelane_b32 %0, %1, m0" : "+v"(x) : "s"(v), "{m0}"(lane));
+    return x;
+}
+
 // Median of a, b and c in one instruction.
 __device__ __forceinline__ int med3_i32(int a, int b, int c) {
     int r;
@@ -449,6 +457,20 @@ __device__ __forceinline__ int eq_le3(const uint32_t* P, const uint32_t* T, int 
     return (int)(z >> 1);
 }
 
+// The same in doubled units: 2v and 2h in, twice the equal bases out.  Twice
+// a position is both its words' funnel shift (low five bits) and, shifted
+// right by five, its first word.
+__device__ __forceinline__ int eq2_le3(const uint32_t* P, const uint32_t* T, int v2, int h2) {
+    const uint32_t* p = P + (v2 >> 5);
+    const uint32_t* t = T + (h2 >> 5);
+    const uint32_t sv = (uint32_t)v2, sh = (uint32_t)h2;
+    const uint32_t xl = __builtin_amdgcn_alignbit(p[1], p[0], sv) ^ __builtin_amdgcn_alignbit(t[1], t[0], sh);
+    const uint32_t xh = __builtin_amdgcn_alignbit(p[2], p[1], sv) ^ __builtin_amdgcn_alignbit(t[2], t[1], sh);
+    const uint64_t x = ((uint64_t)xh << 32) | xl;
+    const uint32_t z = x ? (uint32_t)__builtin_ctzll(x) : 64u;
+    return (int)(z & ~1u);  // base b of the window sits in bits 2b, 2b+1
+}
+
 // Equal bases from (v, h) up to the first mismatch or either end.  Positions
 // past an end extend by nothing; the reads are clamped into the copies.
 __device__ __forceinline__ int extend_le3(const uint32_t* P, const uint32_t* T, int plen, int tlen, int v, int h) {
@@ -499,55 +521,57 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
 
     const int tk = tlen - plen;  // target diagonal, target offset tlen
     const int tka = tk < 0 ? -tk : tk;
+    // offsets are kept doubled (none: -2), positions too, as eq2_le3 takes them
     if (gtid == 0) {
-        X[0] = extend_le3(P32, T32, plen, tlen, 0, 0);
-        X[-1] = -1;
-        X[1] = -1;
-        X[-2] = -1;
-        X[2] = -1;
+        X[0] = 2 * extend_le3(P32, T32, plen, tlen, 0, 0);
+        X[-1] = -2;
+        X[1] = -2;
+        X[-2] = -2;
+        X[2] = -2;
     }
     group_sync<G>();
 
     const int cap = a.max_distance;
     int limit = dcap;
     if (cap > 0 && cap - 1 < limit) limit = cap - 1;
-    bool done = (tk == 0) && (X[0] == tlen);
+    bool done = (tk == 0) && (X[0] == 2 * tlen);
     int d = 0;
     int32_t* cur = X;
     int32_t* nxt = Y;
+    const int lane2 = 2 * lane;
     while (!done && d < limit) {
         ++d;
         const int nch = (d >> 5) + 1;
         const uint32_t base = bt_level_base((uint32_t)d);
         const int32_t* curm1 = cur - 1;
         for (int c = wave; c < nch; c += G) {
-            const int idx = (c << 6) + lane;  // diagonal index 0..2d
-            const int k = idx - d;
+            const int idx2 = (c << 7) + lane2;  // twice the diagonal index 0..2d
             // lane 0 of every chunk up to nch-1 is inside the level
-            if (idx <= 2 * d) {
+            if (idx2 <= 4 * d) {
+                const int k2 = idx2 - 2 * d;  // twice the diagonal
                 // candidates from level d-1 (src/kernels.cu:131-138): DEL
-                // o(k+1), SUB o(k)+1, INS o(k-1)+1; the larger wins, DEL on
-                // ties, then SUB (src/kernels.cu:141-152)
-                const int32_t* q = curm1 + k;  // diagonals k-1, k, k+1
+                // o(k+1), SUB o(k)+1, INS o(k-1)+1, doubled; the larger wins,
+                // DEL on ties, then SUB (src/kernels.cu:141-152)
+                const int32_t* q = curm1 + (k2 >> 1);  // diagonals k-1, k, k+1
                 const int ins = q[0], sub = q[1], del = q[2];
-                const int si = max(sub, ins) + 1;
+                const int si = max(sub, ins) + 2;
                 const int val = max(del, si);
                 const uint64_t lo = __builtin_amdgcn_ballot_w64(del >= si);  // DEL won
                 const uint64_t hi = __builtin_amdgcn_ballot_w64(sub >= ins) & ~lo;  // SUB won
-                const int v = val - k;
+                const int v = val - k2;
                 // matched bases, capped at the bases left (none past an end);
                 // a valid offset is never left of either start, and a read
                 // past an end stays inside the slot
-                const int mc = min(plen + k, tlen);
-                int acc = med3_i32(eq_le3(P32, T32, v, val), mc - val, 0);
-                if (acc == 32) {
+                const int mc = min(2 * plen + k2, 2 * tlen);
+                int acc = med3_i32(eq2_le3(P32, T32, v, val), mc - val, 0);
+                if (acc == 64) {
                     int e;
                     do {
-                        e = min(eq_le3(P32, T32, v + acc, val + acc), mc - val - acc);
+                        e = min(eq2_le3(P32, T32, v + acc, val + acc), mc - val - acc);
                         acc += e;
-                    } while (e == 32);
+                    } while (e == 64);
                 }
-                nxt[k] = val + acc;
+                nxt[k2 >> 1] = val + acc;
                 if (lane == 0) {
                     if constexpr (BT_LDS) {
                         uint64_t* bq = BT + 2 * (size_t)(base + (uint32_t)c);
@@ -563,10 +587,10 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
         // sentinels for the next level's reads at |k| = d+1, d+2
         if (gtid < 4) {
             const int e = gtid < 2 ? -(d + 1 + gtid) : (d + 1 + (gtid - 2));
-            nxt[e] = -1;
+            nxt[e] = -2;
         }
         group_sync<G>();
-        done = (tka <= d) && (nxt[tk] == tlen);
+        done = (tka <= d) && (nxt[tk] == 2 * tlen);
         int32_t* tmp = cur;
         cur = nxt;
         nxt = tmp;
@@ -609,6 +633,22 @@ __device__ __forceinline__ int eq_le(const uint64_t* P, const uint64_t* T, int v
     const uint64_t x = ((uint64_t)xh << 32) | xl;
     const uint32_t z = x ? (uint32_t)__builtin_ctzll(x) : 64u;
     return (int)(z >> 1);
+}
+
+// The same in doubled units: 2v and 2h in, twice the equal bases out.  Twice
+// a position is both its window's funnel shift (low five bits) and, shifted
+// right by five, its 64-bit word.
+__device__ __forceinline__ int eq2_le(const uint64_t* P, const uint64_t* T, int v2, int h2) {
+    const uint64_t* p = P + (v2 >> 5);
+    const uint64_t* t = T + (h2 >> 5);
+    const uint64_t p0 = p[0], p1 = p[1], t0 = t[0], t1 = t[1];
+    const uint32_t xl = __builtin_amdgcn_alignbit((uint32_t)(p0 >> 32), (uint32_t)p0, (uint32_t)v2) ^
+                        __builtin_amdgcn_alignbit((uint32_t)(t0 >> 32), (uint32_t)t0, (uint32_t)h2);
+    const uint32_t xh = __builtin_amdgcn_alignbit((uint32_t)(p1 >> 32), (uint32_t)p1, (uint32_t)v2) ^
+                        __builtin_amdgcn_alignbit((uint32_t)(t1 >> 32), (uint32_t)t1, (uint32_t)h2);
+    const uint64_t x = ((uint64_t)xh << 32) | xl;
+    const uint32_t z = x ? (uint32_t)__builtin_ctzll(x) : 64u;
+    return (int)(z & ~1u);  // base b of the window sits in bits 2b, 2b+1
 }
 
 // Little-endian copy of a len-base sequence from global words g, cooperatively.
@@ -806,6 +846,82 @@ __device__ __forceinline__ void stage_from_regs(uint64_t* P, uint64_t* T, const 
 // before (k-1): two DPP whole-wave right shifts, with the chunk below's
 // lanes 62/63 entering lane 0/1 through readlane.  Levels 32(n-1)..32n-1
 // take n chunks, each range in a loop of its own.  Levels up to 32*NC-1 fit.
+// Op of the diagonal at ballot bit `bit` (0..63, uniform): DEL (3) when its
+// DEL-won bit in L is set, else SUB (2) when its SUB-over-INS bit in S is,
+// else INS (1).  Two scalar bit tests and selects.
+__device__ __forceinline__ int ballot_op(uint64_t L, uint64_t S, int bit) {
+    int op;
+    asm("s_bitcmp1_b64 %2, %3\n\t"
+        "s_cselect_b32 %0, 2, 1\n\t"
+        "s_bitcmp1_b64 %1, %3\n\t"
+        "s_cselect_b32 %0, 3, %0"
+        : "=&s"(op)
+        : "s"(L), "s"(S), "s"(bit)
+        : "scc");
+    return op;
+}
+
+// Backtrace and fused emit of a banded pass.  Level l's op ballots sit in
+// lane l-1 of dl/dh (DEL won, diagonal bit k+band of the low/high half) and
+// sl/sh (SUB at least INS; an op is DEL if its DEL bit is set, else SUB if
+// its SUB bit is, else INS).  Status, distance and deferral as finish_pair.
+__device__ __forceinline__ void finish_pair_band(const AlignArgs& a, uint32_t pidx, int lane, bool done, int d,
+                                                 int limit, int tk, const ewfa_pair_t& pr, const uint8_t* R,
+                                                 int band, bool longrun, int dl, int dh, int sl, int sh) {
+    ewfa_result_t* res = a.results + pidx;
+    const int tka = tk < 0 ? -tk : tk;
+    const int cap = a.max_distance;
+    int status = EWFA_PAIR_OK;
+    int dist = d, start = d;
+    if (!done) {
+        if (cap > 0 && limit == cap - 1) {
+            // the reference's cap: distance = cap, ops of level cap-1 at the
+            // target diagonal (src/kernels.cu:259, :288-303)
+            status = tka > cap ? EWFA_PAIR_UNDEFINED : EWFA_PAIR_CAPPED;
+            dist = cap;
+            start = cap - 1;
+        } else {
+            if (lane == 0) {
+                const uint32_t at = atomicAdd(a.defer_count, 1u);
+                a.defer_list[at] = pidx;
+                res->status = PAIR_DEFERRED;
+            }
+            return;
+        }
+    }
+    uint32_t* ow32 = reinterpret_cast<uint32_t*>(a.ops + (size_t)pidx * a.ops_stride);
+    if (dist > 0) {
+        // steps start-1 down to 0 (op s from level s+1) when the target
+        // diagonal is inside the last level, else none; each 32-bit half of
+        // an ops word (16 steps, step s at bits 2(s%16)) fills from its top
+        // step down, shifting in two bits a step
+        int walked_top = -1;
+        if (tka <= start && start >= 1) {
+            walked_top = (start - 1) >> 4;
+            // the path's diagonal, as a ballot bit (kept in a scalar register)
+            int bit = __builtin_amdgcn_readfirstlane(tk + band);
+            for (int s = start - 1; s >= 0;) {
+                const int lo = s & ~15;
+                uint32_t hw = 0;
+                for (; s >= lo; --s) {
+                    const uint64_t L = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(dh, s) << 32) |
+                                       (uint32_t)__builtin_amdgcn_readlane(dl, s);
+                    const uint64_t S = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(sh, s) << 32) |
+                                       (uint32_t)__builtin_amdgcn_readlane(sl, s);
+                    const int op = ballot_op(L, S, bit);
+                    hw = (hw << 2) + (uint32_t)op;
+                    bit += op - 2;
+                }
+                if (lane == 0) ow32[lo >> 4] = hw;
+            }
+        }
+        // the half words above the walked ones, up to step dist-1, hold zero ops
+        for (int q = 2 * ((dist - 1) >> 5) + 1; q > walked_top; --q)
+            if (lane == 0) ow32[q] = 0;
+    }
+    fused_emit(a, pidx, pr, R, dist, start, status, tk, lane, band, longrun);
+}
+
 // Banded register wavefront: one VGPR of 64 diagonals for passes that search
 // at most L <= 63 levels.  Only the band of cells that can still reach the
 // target diagonal tk within L levels matters: (d, k) with |k - tk| <= L - d.
@@ -815,14 +931,15 @@ __device__ __forceinline__ void stage_from_regs(uint64_t* P, uint64_t* T, const 
 // Over all levels the band spans the L+1 diagonals -floor((L-tk)/2) ..
 // ceil((L+tk)/2), so lane j holds diagonal j - A with A = floor((L - tk)/2)
 // throughout: the previous level's k+1, k and k-1 sit in lanes j+1, j and
-// j-1.  The lanes of diagonals -d..d are computed as in the full wavefront;
-// a lane next to the window's edge misses a source there, but that error
-// moves inward one lane per level and never reaches the band (its front at
-// level d is below lane d-A-1 <= lowest band lane, and likewise at the top
-// edge), so for a pair that reaches its target within L levels the distance,
-// ops and runs along the path are those of the full wavefront.  A pair that
-// does not is deferred (or capped) exactly as before.  Level d's ballots go
-// to BT[2(d-1)], its runs to row 64*d of the run scratch, lane by lane.
+// j-1.  The lanes of diagonals -d..d are computed as in the full wavefront,
+// but for the source outside the wave, read as none; where that diagonal is
+// in the wavefront the error moves inward one lane per level and never
+// reaches the band (its front at level d is below lane d-A-1 <= lowest band
+// lane, and likewise at the top edge), so for a pair that reaches its target
+// within L levels the distance, ops and runs along the path are those of the
+// full wavefront.  A pair that does not is deferred (or capped) exactly as
+// before.  Level d's op ballots go to lane d-1 of four registers, its runs to
+// row 64*d of the run scratch, lane by lane.
 __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pidx, const PairPrefetch& f,
                                                 uint64_t* slot, int lane) {
     const ewfa_pair_t& pr = f.pr;
@@ -833,9 +950,9 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
         if (lane == 0) a.results[pidx] = ewfa_result_t{0, 0, EWFA_PAIR_TOO_LONG, 0};
         return;
     }
-    uint64_t* P = slot;
-    uint64_t* T = slot + sw;
-    uint64_t* BT = slot + 2 * sw;
+    // slot: head guard, pattern copy, text copy, tail guard
+    uint64_t* P = slot + BAND_HEAD_WORDS;
+    uint64_t* T = P + sw;
     if (f.in_regs) {
         stage_from_regs(P, T, f, lane);
     } else {
@@ -859,48 +976,61 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
     bool done = (tk == 0) && (o00 == tlen);
     int d = 0;
     bool longrun = false;  // some level ran a continuation of 7+ steps
+    int dl = 0, dh = 0, sl = 0, sh = 0;  // op ballots, level l in lane l-1
     if (!done && tka <= limit && limit >= 1) {
-        int o = lane == A ? o00 + 1 : 0;  // offset + 1; 0: none
-        const int ca = A - lane;           // v = h + ca on lane's diagonal
-        const int mc = min(plen - ca, tlen);  // bases left from offset h: mc - h
-        const int tl = A + tk;             // lane of the target diagonal
-        uint64_t* BTl = BT;
-        // lanes of the wavefront's diagonals -d..d, clipped to the window
-        uint64_t vm = 1ull << A;
-        int stop = limit;  // one exit: drops to -1 when the target is reached
+        // lanes hold twice the offset (-2: none) and positions are doubled
+        // throughout, as eq2_le takes them
+        int o = lane == A ? 2 * o00 : -2;
+        const int ka = lane < A ? A - lane : lane - A;  // |k|
+        const int ca = 2 * (A - lane);                  // 2v = 2h + ca on lane's diagonal
+        const int mc = 2 * min(plen - (A - lane), tlen);  // twice the bases left from h: mc - 2h
+        const int tl = A + tk;                          // lane of the target diagonal
+        const int want = 2 * tlen;  // a lane outside the wavefront holds -2
+        // DEL and INS sources: lane j+1's and lane j-1's o.  The shifts leave
+        // the lane whose source is outside the wave unwritten, so lane 63's
+        // DEL source and lane 0's INS source stay none (-2) throughout
+        int sdel = -2, sins = -2;
+        const int kb = ka - 1;
+        int lv = 0;  // this level minus one
+        int tv;      // the target diagonal's offset at the level done
         do {
-            ++d;
-            vm |= (vm << 1) | (vm >> 1);
-            const bool valid = __builtin_amdgcn_inverse_ballot_w64(vm);
+            const uint64_t vmask = __builtin_amdgcn_ballot_w64(kb <= lv);  // lanes of diagonals -d..d
+            const bool valid = __builtin_amdgcn_inverse_ballot_w64(vmask);
             // candidates (src/kernels.cu:131-138): DEL from k+1 (lane j+1),
-            // SUB from k (lane j), INS from k-1 (lane j-1), packed as
-            // 4*offset + op so one max gives the offset and DEL(3) > SUB(2) >
-            // INS(1) on ties (src/kernels.cu:141-152)
-            // in stored units the candidates are DEL o(k+1), SUB o(k)+1 and
-            // INS o(k-1)+1; DEL wins ties, then SUB
-            const int sdel = __builtin_amdgcn_update_dpp(0, o, 0x130, 0xF, 0xF, true);
-            const int sins = __builtin_amdgcn_update_dpp(0, o, 0x138, 0xF, 0xF, true);
-            const int si = max(o, sins) + 1;
-            const int st = max(sdel, si);  // the new offset + 1
-            const bool is_del = sdel >= si, is_sub = o >= sins;
-            const int h = st - 1;
-            int acc = 0;
-            if (valid) {
-                // matched bases, capped at the bases left (none past an end)
-                acc = med3_i32(eq_le(P, T, h + ca, h), mc - h, 0);
-            }
-            if (__builtin_amdgcn_ballot_w64(acc == 32)) {
+            // SUB from k (lane j), INS from k-1 (lane j-1): doubled, DEL o(k+1),
+            // SUB o(k)+2, INS o(k-1)+2; the larger wins, DEL on ties, then SUB
+            // (src/kernels.cu:141-152).  A none (-2) gives at most 0, below
+            // every candidate from an offset but a DEL of offset 0, which wins
+            // that tie as it would against a none.
+            sdel = __builtin_amdgcn_update_dpp(sdel, o, 0x130, 0xF, 0xF, false);
+            sins = __builtin_amdgcn_update_dpp(sins, o, 0x138, 0xF, 0xF, false);
+            const int si = max(o, sins) + 2;
+            const int h = max(sdel, si);  // twice the new offset
+            const uint64_t bl = __builtin_amdgcn_ballot_w64(sdel >= si);
+            const uint64_t bs = __builtin_amdgcn_ballot_w64(o >= sins);
+            dl = writelane_i32(dl, (int)(uint32_t)bl, lv);
+            dh = writelane_i32(dh, (int)(uint32_t)(bl >> 32), lv);
+            sl = writelane_i32(sl, (int)(uint32_t)bs, lv);
+            sh = writelane_i32(sh, (int)(uint32_t)(bs >> 32), lv);
+            // twice the matched bases, capped at the bases left (none past an
+            // end).  Every lane compares: one outside the wavefront holds
+            // offset 0, its reads stay inside the slot's guards, and its
+            // result is dropped
+            int acc = med3_i32(eq2_le(P, T, h + ca, h), mc - h, 0);
+            uint64_t cm = __builtin_amdgcn_ballot_w64(acc == 64) & vmask;
+            if (cm) {
                 int last = acc;
                 int hh = h + acc, vv = hh + ca;
                 int it = 0;
                 do {
-                    if (last == 32) {
-                        last = min(eq_le(P, T, vv, hh), mc - hh);
+                    if (__builtin_amdgcn_inverse_ballot_w64(cm)) {
+                        last = min(eq2_le(P, T, vv, hh), mc - hh);
                         hh += last;
                         vv += last;
                     }
                     ++it;
-                } while (__builtin_amdgcn_ballot_w64(last == 32));
+                    cm &= __builtin_amdgcn_ballot_w64(last == 64);
+                } while (cm);
                 acc = hh - h;
                 // a run of 255 bases or more takes 32 + 7 * 32 + more: seven
                 // continuation steps at least
@@ -908,28 +1038,18 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
             }
             // the run's low byte (a pair with a run of 255+ is emitted by the
             // emit kernel, which recomputes the runs)
-            __builtin_amdgcn_raw_buffer_store_b8((uint8_t)acc, Rs, lane, 64 * d, 0);
-            o = valid ? st + acc : 0;
-            // op ballots: DEL won, SUB won (else INS)
-            const uint64_t bl = __builtin_amdgcn_ballot_w64(is_del);
-            const uint64_t bh = __builtin_amdgcn_ballot_w64(is_sub) & ~bl;
-            if (lane == 0) {
-                BTl[0] = bl;
-                BTl[1] = bh;
-            }
-            BTl += 2;
-            // the target diagonal's offset; before level |tk| it is outside
-            // the wavefront, and no offset equals -7
-            const int tv = __builtin_amdgcn_readlane(o, tl);
-            const int want = d >= tka ? tlen + 1 : -7;
-            stop = tv == want ? -1 : stop;
-        } while (d < stop);
-        done = stop < 0;
+            __builtin_amdgcn_raw_buffer_store_b8((uint8_t)(acc >> 1), Rs, lane + 64, 64 * lv, 0);
+            o = valid ? h + acc : -2;
+            ++lv;
+            tv = __builtin_amdgcn_readlane(o, tl);
+        } while ((tv == want ? 64 : lv) < limit);  // limit <= 63
+        d = lv;
+        done = tv == want;
     }
     group_sync<1>();
     // (when |tk| > limit nothing is walked, and row 0 is read where it was
     // written)
-    finish_pair<true>(a, pidx, BT, lane, done, d, limit, tk, &pr, R, min(max(A, 0), 63), longrun);
+    finish_pair_band(a, pidx, lane, done, d, limit, tk, pr, R, min(max(A, 0), 63), longrun, dl, dh, sl, sh);
 }
 
 template <int NC>
