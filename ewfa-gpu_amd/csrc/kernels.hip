@@ -208,6 +208,9 @@ __device__ __forceinline__ uint32_t run_index(int st, int k, int band) {
     return (uint32_t)(64 * st + min(max(kc + band, 0), 63));
 }
 
+// ROUNDS: 2 for paths of up to 128 steps, 1 when the pass has at most 64
+// (distance at most 64).
+template <int ROUNDS>
 __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, const ewfa_pair_t& pr, const uint8_t* R,
                                            int dist, int start, int status, int tk, int lane, int band,
                                            bool longrun) {
@@ -222,18 +225,18 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
     // lie in the words the traceback wrote
     const uint64_t* ow = a.ops + (size_t)pidx * a.ops_stride;
     const int opr0 = lane < dist ? (int)((ow[lane >> 5] >> ((lane & 31) << 1)) & 3u) : 0;
-    const int opr1 = 64 + lane < dist ? (int)((ow[(64 + lane) >> 5] >> ((lane & 31) << 1)) & 3u) : 0;
+    const int opr1 = ROUNDS > 1 && 64 + lane < dist ? (int)((ow[(64 + lane) >> 5] >> ((lane & 31) << 1)) & 3u) : 0;
     const int nsteps = status == EWFA_PAIR_UNDEFINED ? 0 : dist;
     // step s's diagonal: k_s = tk + (sum of op - 2 over steps s..start-1);
     // round 1 (steps 64..127) first, its total carried into round 0.  The walk
     // stops at the first op 0 (a capped path): at most one, in the last round.
     const int x1 = 64 + lane < start ? opr1 - 2 : 0;
-    const int x1_incl = wave_incl_scan_i32(x1);
-    const int tot1 = (int)__builtin_amdgcn_readlane(x1_incl, 63);
+    const int x1_incl = ROUNDS > 1 ? wave_incl_scan_i32(x1) : 0;
+    const int tot1 = ROUNDS > 1 ? (int)__builtin_amdgcn_readlane(x1_incl, 63) : 0;
     int stop = nsteps;
     {
         const uint64_t z0 = __ballot(lane < nsteps && opr0 == 0);
-        const uint64_t z1 = __ballot(64 + lane < nsteps && opr1 == 0);
+        const uint64_t z1 = ROUNDS > 1 ? __ballot(64 + lane < nsteps && opr1 == 0) : 0ull;
         if (z0) stop = __ffsll((long long)z0) - 1;
         else if (z1) stop = 64 + __ffsll((long long)z1) - 1;
     }
@@ -248,7 +251,7 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
     }
     int run0 = 0, run1 = 0;
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < ROUNDS; ++r) {
         const int st = 64 * r + lane;
         const int op = r ? opr1 : opr0;
         const int x = st < start ? op - 2 : 0;
@@ -273,7 +276,7 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
     const uint32_t cap = pr.cigar_cap;
     uint32_t base = 0;
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < ROUNDS; ++r) {
         const int st = 64 * r + lane;
         const bool act = st < nsteps && st <= stop;
         const int op = r ? opr1 : opr0;
@@ -432,7 +435,7 @@ __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, c
         }
     }
     if constexpr (FUSED) {
-        fused_emit(a, pidx, *prp, R, dist, start, status, tk, lane, band, longrun);
+        fused_emit<2>(a, pidx, *prp, R, dist, start, status, tk, lane, band, longrun);
         return;
     }
     if (lane == 0) *res = ewfa_result_t{dist, 0, status, 0};
@@ -840,20 +843,15 @@ __device__ __forceinline__ void stage_from_regs(uint64_t* P, uint64_t* T, const 
     }
 }
 
-// Register-resident wavefront for one wave per pair.  The offsets of a level
-// live in NC VGPRs: chunk c, lane j holds diagonal k = 64c + j - d.  The next
-// level reads the same lane (k+1), the lane before (k) and the lane two
-// before (k-1): two DPP whole-wave right shifts, with the chunk below's
-// lanes 62/63 entering lane 0/1 through readlane.  Levels 32(n-1)..32n-1
-// take n chunks, each range in a loop of its own.  Levels up to 32*NC-1 fit.
 // Op of the diagonal at ballot bit `bit` (0..63, uniform): DEL (3) when its
 // DEL-won bit in L is set, else SUB (2) when its SUB-over-INS bit in S is,
-// else INS (1).  Two scalar bit tests and selects.
-__device__ __forceinline__ int ballot_op(uint64_t L, uint64_t S, int bit) {
+// else INS (1).  Two scalar bit tests and selects on the 32-bit half of the
+// ballots that holds the bit (the test takes the bit index modulo 32).
+__device__ __forceinline__ int ballot_op32(uint32_t L, uint32_t S, int bit) {
     int op;
-    asm("s_bitcmp1_b64 %2, %3\n\t"
+    asm("s_bitcmp1_b32 %2, %3\n\t"
         "s_cselect_b32 %0, 2, 1\n\t"
-        "s_bitcmp1_b64 %1, %3\n\t"
+        "s_bitcmp1_b32 %1, %3\n\t"
         "s_cselect_b32 %0, 3, %0"
         : "=&s"(op)
         : "s"(L), "s"(S), "s"(bit)
@@ -904,11 +902,16 @@ __device__ __forceinline__ void finish_pair_band(const AlignArgs& a, uint32_t pi
                 const int lo = s & ~15;
                 uint32_t hw = 0;
                 for (; s >= lo; --s) {
-                    const uint64_t L = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(dh, s) << 32) |
-                                       (uint32_t)__builtin_amdgcn_readlane(dl, s);
-                    const uint64_t S = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(sh, s) << 32) |
-                                       (uint32_t)__builtin_amdgcn_readlane(sl, s);
-                    const int op = ballot_op(L, S, bit);
+                    // the ballot half that holds the path's bit
+                    uint32_t L, S;
+                    if (bit >= 32) {
+                        L = (uint32_t)__builtin_amdgcn_readlane(dh, s);
+                        S = (uint32_t)__builtin_amdgcn_readlane(sh, s);
+                    } else {
+                        L = (uint32_t)__builtin_amdgcn_readlane(dl, s);
+                        S = (uint32_t)__builtin_amdgcn_readlane(sl, s);
+                    }
+                    const int op = ballot_op32(L, S, bit);
                     hw = (hw << 2) + (uint32_t)op;
                     bit += op - 2;
                 }
@@ -919,7 +922,7 @@ __device__ __forceinline__ void finish_pair_band(const AlignArgs& a, uint32_t pi
         for (int q = 2 * ((dist - 1) >> 5) + 1; q > walked_top; --q)
             if (lane == 0) ow32[q] = 0;
     }
-    fused_emit(a, pidx, pr, R, dist, start, status, tk, lane, band, longrun);
+    fused_emit<1>(a, pidx, pr, R, dist, start, status, tk, lane, band, longrun);  // dist <= 64
 }
 
 // Banded register wavefront: one VGPR of 64 diagonals for passes that search
@@ -992,7 +995,7 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
         int sdel = -2, sins = -2;
         const int kb = ka - 1;
         int lv = 0;  // this level minus one
-        int tv;      // the target diagonal's offset at the level done
+        int stop;
         do {
             const uint64_t vmask = __builtin_amdgcn_ballot_w64(kb <= lv);  // lanes of diagonals -d..d
             const bool valid = __builtin_amdgcn_inverse_ballot_w64(vmask);
@@ -1041,10 +1044,11 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
             __builtin_amdgcn_raw_buffer_store_b8((uint8_t)(acc >> 1), Rs, lane + 64, 64 * lv, 0);
             o = valid ? h + acc : -2;
             ++lv;
-            tv = __builtin_amdgcn_readlane(o, tl);
-        } while ((tv == want ? 64 : lv) < limit);  // limit <= 63
+            // past the limit (<= 63) once the target diagonal holds tlen
+            stop = __builtin_amdgcn_readlane(o, tl) == want ? 64 : lv;
+        } while (stop < limit);
         d = lv;
-        done = tv == want;
+        done = stop == 64;
     }
     group_sync<1>();
     // (when |tk| > limit nothing is walked, and row 0 is read where it was
@@ -1052,6 +1056,12 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
     finish_pair_band(a, pidx, lane, done, d, limit, tk, pr, R, min(max(A, 0), 63), longrun, dl, dh, sl, sh);
 }
 
+// Register-resident wavefront for one wave per pair.  The offsets of a level
+// live in NC VGPRs: chunk c, lane j holds diagonal k = 64c + j - d.  The next
+// level reads the same lane (k+1), the lane before (k) and the lane two
+// before (k-1): two DPP whole-wave right shifts, with the chunk below's
+// lanes 62/63 entering lane 0/1 through readlane.  Levels 32(n-1)..32n-1
+// take n chunks, each range in a loop of its own.  Levels up to 32*NC-1 fit.
 template <int NC>
 __device__ __forceinline__ void align_pair_reg(const AlignArgs& a, uint32_t pidx, const PairPrefetch& f, uint64_t* slot, int lane) {
     if constexpr (NC == 1) {
