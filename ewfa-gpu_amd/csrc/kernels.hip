@@ -984,7 +984,6 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
         // lanes hold twice the offset (-2: none) and positions are doubled
         // throughout, as eq2_le takes them
         int o = lane == A ? 2 * o00 : -2;
-        const int ka = lane < A ? A - lane : lane - A;  // |k|
         const int ca = 2 * (A - lane);                  // 2v = 2h + ca on lane's diagonal
         const int mc = 2 * min(plen - (A - lane), tlen);  // twice the bases left from h: mc - 2h
         const int tl = A + tk;                          // lane of the target diagonal
@@ -993,11 +992,11 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
         // the lane whose source is outside the wave unwritten, so lane 63's
         // DEL source and lane 0's INS source stay none (-2) throughout
         int sdel = -2, sins = -2;
-        const int kb = ka - 1;
+        uint64_t vmask = 1ull << A;  // lanes of diagonals -d..d, clipped to the window
         int lv = 0;  // this level minus one
         int stop;
         do {
-            const uint64_t vmask = __builtin_amdgcn_ballot_w64(kb <= lv);  // lanes of diagonals -d..d
+            vmask |= (vmask << 1) | (vmask >> 1);
             const bool valid = __builtin_amdgcn_inverse_ballot_w64(vmask);
             // candidates (src/kernels.cu:131-138): DEL from k+1 (lane j+1),
             // SUB from k (lane j), INS from k-1 (lane j-1): doubled, DEL o(k+1),
