@@ -330,102 +330,52 @@ __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, c
         uint64_t word = 0;
         int k = tk;
         const int last = tka <= start ? 1 : start + 1;  // no walk when |t| > start
-        // band layout: level l's two ballots at BT[2(l-1)] (DEL won, SUB
-        // won), diagonal k at bit k+band
-        if (band >= 0) {
-            // steps s = start-1 down to last-1, op s from level s+1; each
-            // 32-bit half of an ops word (16 steps) fills from its top step
-            // down, shifting in two bits a step
-            uint32_t* ow32 = reinterpret_cast<uint32_t*>(ow);
-            uint32_t hw = 0;  // the half word being filled
-            for (int top = start; top >= last; top -= 64) {
-                const int lev_l = top - lane;
-                uint64_t rl = 0, rh = 0;
-                if (lev_l >= 1) {
-                    rl = BT[2 * (lev_l - 1)];
-                    rh = BT[2 * (lev_l - 1) + 1];
-                }
-                const int s_end = max(top - 64, last - 1);  // lowest step of the window
-                int s = top - 1;
-                while (s >= s_end) {
-                    // the steps of this window inside half word s>>4
-                    const int w_end = max(s_end, s & ~15);
-                    for (; s >= w_end; --s) {
-                        const int j = top - 1 - s;
-                        const int bit = k + band;
-                        uint32_t wl, wh;
-                        if (bit >= 32) {
-                            wl = __builtin_amdgcn_readlane((int)(uint32_t)(rl >> 32), j);
-                            wh = __builtin_amdgcn_readlane((int)(uint32_t)(rh >> 32), j);
-                        } else {
-                            wl = __builtin_amdgcn_readlane((int)(uint32_t)rl, j);
-                            wh = __builtin_amdgcn_readlane((int)(uint32_t)rh, j);
-                        }
-                        const int b = bit & 31;
-                        // (DEL won, SUB won) -> DEL 3, SUB 2, else INS 1
-                        const int op = ((wl >> b) & 1u) ? OP_DEL : ((wh >> b) & 1u) ? OP_SUB : OP_INS;
-                        hw = (hw << 2) | (uint32_t)op;
-                        k += op - 2;
+        for (int top = start; top >= last; top -= 64) {
+            const int lev_l = top - lane;
+            const int c0 = max(0, (k + top - 2 * lane) >> 6);
+            uint64_t r[3][2];
+#pragma unroll
+            for (int u = 0; u < 3; ++u) r[u][0] = r[u][1] = 0;
+            if (lev_l >= 1) {
+                const uint64_t* row = BT + 2 * (size_t)bt_level_base((uint32_t)lev_l);
+                const int nch = (lev_l >> 5) + 1;
+#pragma unroll
+                for (int u = 0; u < 3; ++u)
+                    if (c0 + u < nch) {
+                        r[u][0] = row[2 * (c0 + u)];
+                        r[u][1] = row[2 * (c0 + u) + 1];
                     }
-                    if ((s & 15) == 15) {  // step s+1 opened its half word: complete
-                        if (lane == 0) ow32[(s + 1) >> 4] = hw;
-                        hw = 0;
-                    }
-                }
             }
-            // the walk is all steps down to 0 (last == 1) or none; the half
-            // words above the walked ones, up to step dist-1, hold zero ops
-            const int walked_top = (last == 1 && start >= 1) ? ((start - 1) >> 4) : -1;
-            for (int q = 2 * wi + 1; q > walked_top; --q)
-                if (lane == 0) ow32[q] = 0;
-            wi = -1;
-        } else {
-            for (int top = start; top >= last; top -= 64) {
-                const int lev_l = top - lane;
-                const int c0 = max(0, (k + top - 2 * lane) >> 6);
-                uint64_t r[3][2];
-#pragma unroll
-                for (int u = 0; u < 3; ++u) r[u][0] = r[u][1] = 0;
-                if (lev_l >= 1) {
-                    const uint64_t* row = BT + 2 * (size_t)bt_level_base((uint32_t)lev_l);
-                    const int nch = (lev_l >> 5) + 1;
-#pragma unroll
-                    for (int u = 0; u < 3; ++u)
-                        if (c0 + u < nch) {
-                            r[u][0] = row[2 * (c0 + u)];
-                            r[u][1] = row[2 * (c0 + u) + 1];
-                        }
+            const int steps = min(64, top - last + 1);
+            for (int j = 0; j < steps; ++j) {
+                const int lev = top - j;
+                const int s = lev - 1;
+                while (wi > (s >> 5)) {
+                    if (lane == 0) ow[wi] = word;
+                    --wi;
+                    word = 0;
                 }
-                const int steps = min(64, top - last + 1);
-                for (int j = 0; j < steps; ++j) {
-                    const int lev = top - j;
-                    const int s = lev - 1;
-                    while (wi > (s >> 5)) {
-                        if (lane == 0) ow[wi] = word;
-                        --wi;
-                        word = 0;
+                const int idx = k + lev;
+                const int rel = (idx >> 6) - __builtin_amdgcn_readlane(c0, j);
+                const int bit = idx & 63;
+                uint32_t wl = 0, wh = 0;
+#pragma unroll
+                for (int u = 0; u < 3; ++u)
+                    if (rel == u) {
+                        if (bit >= 32) {
+                            wl = __builtin_amdgcn_readlane((int)(uint32_t)(r[u][0] >> 32), j);
+                            wh = __builtin_amdgcn_readlane((int)(uint32_t)(r[u][1] >> 32), j);
+                        } else {
+                            wl = __builtin_amdgcn_readlane((int)(uint32_t)r[u][0], j);
+                            wh = __builtin_amdgcn_readlane((int)(uint32_t)r[u][1], j);
+                        }
                     }
-                    const int idx = k + lev;
-                    const int rel = (idx >> 6) - __builtin_amdgcn_readlane(c0, j);
-                    const int bit = idx & 63;
-                    uint32_t wl = 0, wh = 0;
-#pragma unroll
-                    for (int u = 0; u < 3; ++u)
-                        if (rel == u) {
-                            if (bit >= 32) {
-                                wl = __builtin_amdgcn_readlane((int)(uint32_t)(r[u][0] >> 32), j);
-                                wh = __builtin_amdgcn_readlane((int)(uint32_t)(r[u][1] >> 32), j);
-                            } else {
-                                wl = __builtin_amdgcn_readlane((int)(uint32_t)r[u][0], j);
-                                wh = __builtin_amdgcn_readlane((int)(uint32_t)r[u][1], j);
-                            }
-                        }
-                    const int b = bit & 31;
-                    // (DEL won, SUB won) -> DEL 3, SUB 2, else INS 1
-                    const int op = ((wl >> b) & 1u) ? OP_DEL : ((wh >> b) & 1u) ? OP_SUB : OP_INS;
-                    word |= (uint64_t)op << ((s & 31) << 1);
-                    k += op - 2;
-                }
+                const int b = bit & 31;
+                // DEL when its DEL-won bit is set, else SUB when its SUB bit
+                // is (SUB won, or SUB at least INS), else INS
+                const int op = ((wl >> b) & 1u) ? OP_DEL : ((wh >> b) & 1u) ? OP_SUB : OP_INS;
+                word |= (uint64_t)op << ((s & 31) << 1);
+                k += op - 2;
             }
         }
         while (wi >= 0) {
@@ -542,10 +492,10 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
     int32_t* cur = X;
     int32_t* nxt = Y;
     const int lane2 = 2 * lane;
+    uint32_t base = 0;  // bt_level_base(d): level d's first chunk
     while (!done && d < limit) {
         ++d;
         const int nch = (d >> 5) + 1;
-        const uint32_t base = bt_level_base((uint32_t)d);
         const int32_t* curm1 = cur - 1;
         for (int c = wave; c < nch; c += G) {
             const int idx2 = (c << 7) + lane2;  // twice the diagonal index 0..2d
@@ -559,8 +509,10 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
                 const int ins = q[0], sub = q[1], del = q[2];
                 const int si = max(sub, ins) + 2;
                 const int val = max(del, si);
-                const uint64_t lo = __builtin_amdgcn_ballot_w64(del >= si);  // DEL won
-                const uint64_t hi = __builtin_amdgcn_ballot_w64(sub >= ins) & ~lo;  // SUB won
+                // op ballots: DEL won; SUB at least INS (the op is DEL when
+                // its DEL bit is set, else SUB when this bit is, else INS)
+                const uint64_t lo = __builtin_amdgcn_ballot_w64(del >= si);
+                const uint64_t hi = __builtin_amdgcn_ballot_w64(sub >= ins);
                 const int v = val - k2;
                 // matched bases, capped at the bases left (none past an end);
                 // a valid offset is never left of either start, and a read
@@ -574,7 +526,8 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
                         acc += e;
                     } while (e == 64);
                 }
-                nxt[k2 >> 1] = val + acc;
+                // nxt[k]: k2 is even, so 2*k2 is the byte offset
+                *reinterpret_cast<int32_t*>(reinterpret_cast<char*>(nxt) + 2 * k2) = val + acc;
                 if (lane == 0) {
                     if constexpr (BT_LDS) {
                         uint64_t* bq = BT + 2 * (size_t)(base + (uint32_t)c);
@@ -593,6 +546,7 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
             nxt[e] = -2;
         }
         group_sync<G>();
+        base += (uint32_t)nch;
         done = (tka <= d) && (nxt[tk] == 2 * tlen);
         int32_t* tmp = cur;
         cur = nxt;
