@@ -21,7 +21,7 @@ Policy default_policy() {
     // tuned on 500,000 pairs of log-uniform length 100..10,000 bp at 1..10%
     // edits; longer pairs take a 16-wave block with as many levels as LDS holds
     //              len_hi   waves dcap  bt_lds grab
-    p.buckets.push_back({300u, 1, 127, 1, 16});
+    p.buckets.push_back({300u, 1, 63, 1, 16});
     p.buckets.push_back({700u, 1, 127, 1, 4});
     p.buckets.push_back({1200u, 1, 127, 1, 4});
     p.buckets.push_back({5000u, 2, 512, 0, 1});
