@@ -106,7 +106,7 @@ def oracle_streams(P, T):
 
 @pytest.mark.parametrize("seed", [101, 202])
 def test_low_complexity_pairs_all_policies(tmp_path, seed):
-    P, T = stress_pairs(seed, 700)
+    P, T = stress_pairs(seed, 1500)
     d, st = oracle_streams(P, T)
     words, pairs, cb, ml = ewfa.from_sequences(P, T)
     ref = None
