@@ -215,7 +215,9 @@ int pass_from_bucket(ewfa_ctx* c, const ewfa::BucketPolicy& b, uint32_t blen, ui
     // one wave with the backtrace in LDS: offsets in registers when they fit
     if (p.waves == 1 && p.bt_lds) {
         const int nc = ewfa::reg_chunks_for(p.dcap);
-        const int ow = overlapped_words_for(blen);
+        // the banded kernel (one chunk) keeps plain copies, the others
+        // overlapped ones
+        const int ow = nc == 1 ? seq_words_for(blen) : overlapped_words_for(blen);
         if (nc > 0 && ewfa::lds_bytes_reg(ow, p.dcap, nc) <= 65536) {
             p.reg = nc;
             p.seq_words = ow;
@@ -338,8 +340,9 @@ int reserve_scratch(ewfa_ctx* c, size_t n, uint32_t max_len) {
 }
 
 // Run scratch of a register pass with the emit fused (bytes per wave).
+// The banded kernel keeps its run rows in LDS.
 uint32_t runs_slot_of(const Pass& p) {
-    if (p.reg == 1) return ewfa::runs_slot_bytes_band((uint32_t)std::min(p.dcap, 63));
+    if (p.reg == 1) return 0u;
     return ewfa::runs_slot_bytes((uint32_t)std::min(p.dcap, 32 * p.reg - 1));
 }
 

@@ -109,17 +109,20 @@ inline size_t lds_bytes(int seq_words, int dcap, int bt_lds) {
     return b;
 }
 
-// Guard words of a banded slot before the pattern copy and after the text
-// copy: reads of lanes outside the wavefront, and of offsets past an end,
-// land in them.
-constexpr int BAND_HEAD_WORDS = 4;
-constexpr int BAND_TAIL_WORDS = 8;
+// Guard words (64-bit) of a banded slot before the pattern copy and after
+// the text copy: reads of lanes outside the wavefront, and of offsets past an
+// end, land in them.
+constexpr int BAND_HEAD_WORDS = 2;
+constexpr int BAND_TAIL_WORDS = 4;
+// The banded slot's run rows: a 64-byte row per level 0..63.
+constexpr int BAND_RUN_BYTES = 64 * 64;
 
-// LDS bytes of a register-wavefront slot (sequences + backtrace only).  One
-// chunk (the banded wavefront) keeps its backtrace in registers: the slot is
-// the two copies between the guards.
+// LDS bytes of a register-wavefront slot.  One chunk (the banded wavefront)
+// keeps its backtrace in registers: its slot is the two plain copies
+// (seq_words each) between the guards, then the run rows.  More chunks: the
+// overlapped copies and the backtrace.
 inline size_t lds_bytes_reg(int seq_words, int dcap, int nc) {
-    if (nc == 1) return 8u * (size_t)(2 * seq_words + BAND_HEAD_WORDS + BAND_TAIL_WORDS);
+    if (nc == 1) return 8u * (size_t)(2 * seq_words + BAND_HEAD_WORDS + BAND_TAIL_WORDS) + BAND_RUN_BYTES;
     return 16u * (size_t)seq_words + 16u * (size_t)bt_chunks((uint32_t)dcap);
 }
 
@@ -131,9 +134,6 @@ inline int reg_chunks_for(int dcap) {
     return nc <= 4 ? nc : 0;
 }
 
-// Bytes of a wave's run scratch in the banded wavefront: a 64-byte row per
-// level 0..dmax.
-__host__ __device__ inline uint32_t runs_slot_bytes_band(uint32_t dmax) { return ((dmax + 1) * 64u + 127u) & ~127u; }
 
 // Bytes of a pair's CIGAR region the emit kernel wrote: whole dwords from
 // cigar_off up to and including the terminator (or up to cigar_cap when the

@@ -592,22 +592,6 @@ __device__ __forceinline__ int eq_le(const uint64_t* P, const uint64_t* T, int v
     return (int)(z >> 1);
 }
 
-// The same in doubled units: 2v and 2h in, twice the equal bases out.  Twice
-// a position is both its window's funnel shift (low five bits) and, shifted
-// right by five, its 64-bit word.
-__device__ __forceinline__ int eq2_le(const uint64_t* P, const uint64_t* T, int v2, int h2) {
-    const uint64_t* p = P + (v2 >> 5);
-    const uint64_t* t = T + (h2 >> 5);
-    const uint64_t p0 = p[0], p1 = p[1], t0 = t[0], t1 = t[1];
-    const uint32_t xl = __builtin_amdgcn_alignbit((uint32_t)(p0 >> 32), (uint32_t)p0, (uint32_t)v2) ^
-                        __builtin_amdgcn_alignbit((uint32_t)(t0 >> 32), (uint32_t)t0, (uint32_t)h2);
-    const uint32_t xh = __builtin_amdgcn_alignbit((uint32_t)(p1 >> 32), (uint32_t)p1, (uint32_t)v2) ^
-                        __builtin_amdgcn_alignbit((uint32_t)(t1 >> 32), (uint32_t)t1, (uint32_t)h2);
-    const uint64_t x = ((uint64_t)xh << 32) | xl;
-    const uint32_t z = x ? (uint32_t)__builtin_ctzll(x) : 64u;
-    return (int)(z & ~1u);  // base b of the window sits in bits 2b, 2b+1
-}
-
 // Little-endian copy of a len-base sequence from global words g, cooperatively.
 __device__ __forceinline__ void stage_le(uint64_t* S, const uint64_t* g, int len, int lane) {
     const int nw = (len + 31) >> 5;
@@ -768,8 +752,10 @@ __device__ __forceinline__ PairPrefetch prefetch_pair(const AlignArgs& a, uint32
     f.in_regs = (nwp + nwt <= 64) ? 1 : 0;
     f.word = 0;
     if (f.in_regs) {
-        if (lane < nwp) f.word = a.words[f.pr.pattern_off + lane];
-        else if (lane < nwp + nwt) f.word = a.words[f.pr.text_off + (lane - nwp)];
+        // read once: non-temporal, so the streamed words do not displace the
+        // waves' run scratch in L2
+        if (lane < nwp) f.word = __builtin_nontemporal_load(a.words + f.pr.pattern_off + lane);
+        else if (lane < nwp + nwt) f.word = __builtin_nontemporal_load(a.words + f.pr.text_off + (lane - nwp));
     }
     return f;
 }
@@ -795,6 +781,24 @@ __device__ __forceinline__ void stage_from_regs(uint64_t* P, uint64_t* T, const 
         P[2 * nwp + lane] = 0;
         T[2 * nwt + lane] = 0;
     }
+}
+
+// Plain little-endian copies (64-bit word m = the bit reversal of global word
+// m: bases 32m..32m+15 in its low 32-bit half, then 32m+16..32m+31) of both
+// sequences from prefetched registers, each followed by two zero words.
+__device__ __forceinline__ void stage_plain_from_regs(uint64_t* P, uint64_t* T, const PairPrefetch& f, int lane) {
+    const int nwp = ((int)f.pr.plen + 31) >> 5, nwt = ((int)f.pr.tlen + 31) >> 5;
+    const uint64_t w = __builtin_bitreverse64(f.word);
+    if (lane < nwp) P[lane] = w;
+    else if (lane < nwp + nwt) T[lane - nwp] = w;
+    if (lane < 2) P[nwp + lane] = 0ull;
+    else if (lane < 4) T[nwt + lane - 2] = 0ull;
+}
+
+// The same from global words, cooperatively.
+__device__ __forceinline__ void stage_plain(uint64_t* S, const uint64_t* g, int len, int lane) {
+    const int nw = (len + 31) >> 5;
+    for (int i = lane; i < nw + 2; i += 64) S[i] = i < nw ? __builtin_bitreverse64(g[i]) : 0ull;
 }
 
 // Op of the diagonal at ballot bit `bit` (0..63, uniform): DEL (3) when its
@@ -896,25 +900,29 @@ __device__ __forceinline__ void finish_pair_band(const AlignArgs& a, uint32_t pi
 // within L levels the distance, ops and runs along the path are those of the
 // full wavefront.  A pair that does not is deferred (or capped) exactly as
 // before.  Level d's op ballots go to lane d-1 of four registers, its runs to
-// row 64*d of the run scratch, lane by lane.
+// row d of the run rows in LDS, lane by lane.
 __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pidx, const PairPrefetch& f,
                                                 uint64_t* slot, int lane) {
     const ewfa_pair_t& pr = f.pr;
     const int plen = (int)pr.plen, tlen = (int)pr.tlen;
     const int sw = a.seq_words;
     const int nwp = (plen + 31) >> 5, nwt = (tlen + 31) >> 5;
-    if (pr.plen > a.max_len || pr.tlen > a.max_len || 2 * nwp + 2 > sw || 2 * nwt + 2 > sw) {
+    if (pr.plen > a.max_len || pr.tlen > a.max_len || nwp + 2 > sw || nwt + 2 > sw) {
         if (lane == 0) a.results[pidx] = ewfa_result_t{0, 0, EWFA_PAIR_TOO_LONG, 0};
         return;
     }
-    // slot: head guard, pattern copy, text copy, tail guard
+    // slot: head guard, pattern copy, text copy (plain little-endian words),
+    // tail guard, then a 64-byte run row per level
     uint64_t* P = slot + BAND_HEAD_WORDS;
     uint64_t* T = P + sw;
+    const uint32_t* P32 = reinterpret_cast<const uint32_t*>(P);
+    const uint32_t* T32 = reinterpret_cast<const uint32_t*>(T);
+    uint8_t* R = reinterpret_cast<uint8_t*>(T + sw + BAND_TAIL_WORDS);
     if (f.in_regs) {
-        stage_from_regs(P, T, f, lane);
+        stage_plain_from_regs(P, T, f, lane);
     } else {
-        stage_le(P, a.words + pr.pattern_off, plen, lane);
-        stage_le(T, a.words + pr.text_off, tlen, lane);
+        stage_plain(P, a.words + pr.pattern_off, plen, lane);
+        stage_plain(T, a.words + pr.text_off, tlen, lane);
     }
     group_sync<1>();
 
@@ -924,11 +932,7 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
     const int cap = a.max_distance;
     if (cap > 0 && cap - 1 < limit) limit = cap - 1;
     const int A = (limit - tk) >> 1;  // lane of diagonal 0 (when |tk| <= limit)
-    const int o00 = extend_le(P, T, plen, tlen, 0, 0);
-    // run scratch of this wave (the emit is fused), also as a buffer whose
-    // size bounds the level loop's stores to the slot
-    uint8_t* R = a.runs + (size_t)blockIdx.x * a.runs_slot;
-    const __amdgpu_buffer_rsrc_t Rs = __builtin_amdgcn_make_buffer_rsrc(R, (short)0, (int)a.runs_slot, BUF_DWORD3);
+    const int o00 = extend_le3(P32, T32, plen, tlen, 0, 0);
     if (lane == 0) R[min(max(A, 0), 63)] = (uint8_t)min(o00, 255);
     bool done = (tk == 0) && (o00 == tlen);
     int d = 0;
@@ -936,7 +940,7 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
     int dl = 0, dh = 0, sl = 0, sh = 0;  // op ballots, level l in lane l-1
     if (!done && tka <= limit && limit >= 1) {
         // lanes hold twice the offset (-2: none) and positions are doubled
-        // throughout, as eq2_le takes them
+        // throughout, as eq2_le3 takes them
         int o = lane == A ? 2 * o00 : -2;
         const int ca = 2 * (A - lane);                  // 2v = 2h + ca on lane's diagonal
         const int mc = 2 * min(plen - (A - lane), tlen);  // twice the bases left from h: mc - 2h
@@ -972,7 +976,7 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
             // end).  Every lane compares: one outside the wavefront holds
             // offset 0, its reads stay inside the slot's guards, and its
             // result is dropped
-            int acc = med3_i32(eq2_le(P, T, h + ca, h), mc - h, 0);
+            int acc = med3_i32(eq2_le3(P32, T32, h + ca, h), mc - h, 0);
             uint64_t cm = __builtin_amdgcn_ballot_w64(acc == 64) & vmask;
             if (cm) {
                 int last = acc;
@@ -980,7 +984,7 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
                 int it = 0;
                 do {
                     if (__builtin_amdgcn_inverse_ballot_w64(cm)) {
-                        last = min(eq2_le(P, T, vv, hh), mc - hh);
+                        last = min(eq2_le3(P32, T32, vv, hh), mc - hh);
                         hh += last;
                         vv += last;
                     }
@@ -992,9 +996,9 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
                 // continuation steps at least
                 longrun |= it >= 7;
             }
-            // the run's low byte (a pair with a run of 255+ is emitted by the
-            // emit kernel, which recomputes the runs)
-            __builtin_amdgcn_raw_buffer_store_b8((uint8_t)(acc >> 1), Rs, lane + 64, 64 * lv, 0);
+            // the run's low byte in row lv+1 (a pair with a run of 255+ is
+            // emitted by the emit kernel, which recomputes the runs)
+            R[64 * lv + 64 + lane] = (uint8_t)(acc >> 1);
             o = valid ? h + acc : -2;
             ++lv;
             // past the limit (<= 63) once the target diagonal holds tlen
