@@ -1192,9 +1192,15 @@ __device__ __forceinline__ void emit_one(const EmitArgs& a, uint32_t i);
 
 // Grid-stride over the items, so a launch sized for the largest count can
 // run over a count that is only known on the device.
+// One lane per pair, EMIT_PER_WAVE pairs per wave (lanes 0..EMIT_PER_WAVE-1),
+// grid-stride over the items.
 __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a) {
     const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
-    for (uint32_t item = blockIdx.x * 256u + threadIdx.x; item < count; item += gridDim.x * 256u)
+    const uint32_t lane = threadIdx.x & 63u;
+    if (lane >= (uint32_t)EMIT_PER_WAVE) return;
+    const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t stride = gridDim.x * 4u * (uint32_t)EMIT_PER_WAVE;
+    for (uint32_t item = wave * (uint32_t)EMIT_PER_WAVE + lane; item < count; item += stride)
         emit_one(a, a.list ? a.list[item] : item);
 }
 
@@ -1752,8 +1758,9 @@ int align_blocks_per_cu(int waves, size_t lds) {
 
 hipError_t launch_emit(const EmitArgs& a, uint32_t grid_items, hipStream_t stream) {
     if (grid_items == 0) return hipSuccess;
-    const uint32_t blocks = (grid_items + 255) / 256;
-    hipLaunchKernelGGL(emit_kernel, dim3(blocks < 8192u ? blocks : 8192u), dim3(256), 0, stream, a);
+    const uint32_t per_block = 4u * (uint32_t)EMIT_PER_WAVE;
+    const uint32_t blocks = (grid_items + per_block - 1) / per_block;
+    hipLaunchKernelGGL(emit_kernel, dim3(blocks < 16384u ? blocks : 16384u), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 
