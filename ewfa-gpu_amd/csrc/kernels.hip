@@ -1415,12 +1415,17 @@ __global__ __launch_bounds__(64) void emit_lds_kernel(EmitArgs a, int sw) {
 // length buckets
 
 // Sorts pair indices into per-bucket lists by max(plen, tlen): bucket b takes
-// lengths up to hi[b] (the last bucket takes the rest).  One atomic per bucket
-// per wave; the order inside a bucket does not matter for the results.
-__global__ __launch_bounds__(256) void bucketize_kernel(const ewfa_pair_t* pairs, uint32_t n, BucketTable t,
-                                                        uint32_t* lists, uint32_t* counts) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+// lengths up to hi[b] (the last bucket takes the rest).  A block of 1,024
+// pairs counts its members per bucket in LDS (one LDS atomic per bucket per
+// wave), then takes its ranges of the lists with one global atomic per bucket
+// per block; the order inside a bucket does not matter for the results.
+__global__ __launch_bounds__(1024) void bucketize_kernel(const ewfa_pair_t* pairs, uint32_t n, BucketTable t,
+                                                         uint32_t* lists, uint32_t* counts) {
+    __shared__ uint32_t cnt[MAX_BUCKETS], base[MAX_BUCKETS];
+    const uint32_t i = blockIdx.x * 1024u + threadIdx.x;
     const int lane = threadIdx.x & 63;
+    if (threadIdx.x < (uint32_t)t.nb) cnt[threadIdx.x] = 0u;
+    __syncthreads();
     int b = -1;
     if (i < n) {
         const uint32_t L = max(pairs[i].plen, pairs[i].tlen);
@@ -1428,15 +1433,23 @@ __global__ __launch_bounds__(256) void bucketize_kernel(const ewfa_pair_t* pairs
         for (int q = t.nb - 2; q >= 0; --q)
             if (L <= t.hi[q]) b = q;
     }
+    uint32_t local = 0;  // this pair's place among the block's members of bucket b
     for (int q = 0; q < t.nb; ++q) {
         const uint64_t m = __ballot(b == q);
         if (m == 0) continue;
         const int leader = __ffsll((long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(counts + q, (uint32_t)__popcll(m));
-        base = (uint32_t)__shfl((int)base, leader);
-        if (b == q) lists[(size_t)q * n + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+        uint32_t at = 0;
+        if (lane == leader) at = atomicAdd(cnt + q, (uint32_t)__popcll(m));
+        at = (uint32_t)__shfl((int)at, leader);
+        if (b == q) local = at + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)t.nb) {
+        const uint32_t c = cnt[threadIdx.x];
+        base[threadIdx.x] = c ? atomicAdd(counts + threadIdx.x, c) : 0u;
+    }
+    __syncthreads();
+    if (b >= 0) lists[(size_t)b * n + base[b] + local] = i;
 }
 
 // Match run from pattern position v, text position h with the whole wave:
@@ -1799,7 +1812,7 @@ hipError_t launch_pack(const char* ascii, size_t ascii_stride, const uint32_t* l
 hipError_t launch_bucketize(const ewfa_pair_t* pairs, uint32_t n, const BucketTable& t, uint32_t* lists,
                             uint32_t* counts, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(bucketize_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, pairs, n, t, lists, counts);
+    hipLaunchKernelGGL(bucketize_kernel, dim3((n + 1023) / 1024), dim3(1024), 0, stream, pairs, n, t, lists, counts);
     return hipGetLastError();
 }
 
