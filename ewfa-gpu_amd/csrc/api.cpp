@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -43,6 +44,10 @@ int fail(int code, const char* fmt, ...) {
         if (r_ != EWFA_OK) return r_; \
     } while (0)
 
+// Bumped whenever a device buffer is replaced: a captured graph that may
+// hold the old address is captured again.
+std::atomic<uint64_t> g_buffer_gen{0};
+
 // Device buffer that only grows.
 struct DevBuf {
     void* p = nullptr;
@@ -61,6 +66,7 @@ struct DevBuf {
             return fail(EWFA_ERR_NOMEM, "hipMalloc(%zu): %s", n, hipGetErrorString(e));
         }
         bytes = n;
+        g_buffer_gen.fetch_add(1, std::memory_order_relaxed);
         return EWFA_OK;
     }
     void release() {
@@ -78,8 +84,10 @@ enum : int {
     CTRL_DEFER_A = 1,   // deferred-pair counts (ping-pong)
     CTRL_DEFER_B = 2,
     CTRL_FALLBACK = 3,  // pairs the fused emit left to the emit kernel
+    CTRL_CHAIN_BQ = 4,  // work counter of a deferral pass's distance bounds
     CTRL_BQUEUE = 8,    // queue heads of the bucket passes (MAX_BUCKETS)
     CTRL_BCOUNT = 16,   // pairs per bucket (MAX_BUCKETS)
+    CTRL_BOUNDQ = 24,   // work counters of the bucket passes' distance bounds (MAX_BUCKETS)
     CTRL_WORDS = 64
 };
 
@@ -94,7 +102,7 @@ struct ewfa_ctx {
     hipEvent_t ev_in = nullptr, ev_a0 = nullptr, ev_a1 = nullptr, ev_e1 = nullptr;
     int num_cus = 0;
     size_t lds_max = 0;
-    DevBuf ops, bt, ctrl, list_a, list_b, buckets, runs, list_fb;
+    DevBuf ops, bt, ctrl, list_a, list_b, buckets, runs, list_fb, bound;
     uint32_t* host_ctrl = nullptr;  // pinned mirror of the control words
     float align_ms = 0.f, emit_ms = 0.f;
     bool timing = false;
@@ -113,6 +121,7 @@ struct ewfa_ctx {
     } graph_key{};
     bool use_graph = true;
     int policy_gen = 0, graph_policy_gen = -1;
+    uint64_t graph_buffer_gen = 0;  // g_buffer_gen when the graph was captured
     // op scratch supplied by the caller for one call (the job server's arena)
     uint64_t* ops_ext = nullptr;
     size_t ops_ext_bytes = 0;
@@ -357,6 +366,7 @@ int reserve_plan(ewfa_ctx* c, const Plan& pl, size_t n) {
     for (int b = 0; b < pl.nb; ++b) CHK(reserve_bt(c, pl.pass[b]));
     for (int k = 0; k < pl.chain.np; ++k) CHK(reserve_bt(c, pl.chain.pass[k]));
     if (pl.nb > 1) CHK(c->buckets.reserve((size_t)pl.nb * std::max<size_t>(n, 1) * 4));
+    CHK(c->bound.reserve(std::max<size_t>(n, 1) * 4));
     return EWFA_OK;
 }
 
@@ -365,6 +375,29 @@ int launch_pass(const ewfa::AlignArgs& a, const Pass& p, hipStream_t s) {
         HIPCHK(ewfa::launch_align_reg(a, p.reg, p.persistent, p.grid, p.lds, s));
     else
         HIPCHK(ewfa::launch_align(a, p.waves, p.persistent, p.grid, p.lds, s));
+    return EWFA_OK;
+}
+
+// Ahead of a deep block pass: each of its pairs' distance bound from the
+// banded search (same list, level limit and staging), for the pass to leave
+// out the cells no path within the bound uses.  The saving grows with the
+// distance while the bound's cost grows with the levels only: passes with
+// budgets under BOUND_MIN_DCAP levels (where the measured saving did not pay
+// for the bound), register passes, and sequences whose staging exceeds
+// 64 KiB of LDS go without.  `queue` must hold 0.
+constexpr int BOUND_MIN_DCAP = 2048;
+
+int enqueue_bound(ewfa_ctx* c, ewfa::AlignArgs& a, const Pass& p, uint32_t* queue, uint32_t n, hipStream_t s) {
+    a.bound = nullptr;
+    const size_t lds = ewfa::lds_bytes_bound(p.seq_words);
+    if (p.reg || p.dcap < BOUND_MIN_DCAP || lds > 65536 || n == 0) return EWFA_OK;
+    ewfa::AlignArgs b = a;
+    b.queue = queue;
+    b.bound = c->bound.as<uint32_t>();
+    const size_t per_cu = std::max<size_t>(1, std::min<size_t>(16, lds_budget(c) / lds));
+    const uint32_t grid = (uint32_t)std::min<size_t>((size_t)c->num_cus * per_cu, n);
+    HIPCHK(ewfa::launch_bound(b, grid, lds, s));
+    a.bound = b.bound;
     return EWFA_OK;
 }
 
@@ -479,6 +512,8 @@ int enqueue_chain(ewfa_ctx* c, const Chain& ch, const uint64_t* words, const ewf
         a.queue = ctrl + CTRL_QUEUE;
         a.defer_list = lists[1 - cur];
         a.defer_count = ctrl + ctr[1 - cur];
+        HIPCHK(ewfa::launch_fill_u32(ctrl + CTRL_CHAIN_BQ, 0u, 1, s));
+        CHK(enqueue_bound(c, a, p, ctrl + CTRL_CHAIN_BQ, n, s));
         CHK(launch_pass(a, p, s));
         ewfa::EmitArgs e{};
         e.words = words;
@@ -508,7 +543,7 @@ int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_
     uint32_t* lists = c->buckets.as<uint32_t>();
     const bool bucketed = pl.nb > 1;
     if (phase & PLAN_ALIGN) {
-        HIPCHK(ewfa::launch_fill_u32(ctrl, 0u, CTRL_BCOUNT + ewfa::MAX_BUCKETS, s));
+        HIPCHK(ewfa::launch_fill_u32(ctrl, 0u, CTRL_BOUNDQ + ewfa::MAX_BUCKETS, s));
         if (bucketed) HIPCHK(ewfa::launch_bucketize(pairs, n, pl.table, lists, ctrl + CTRL_BCOUNT, s));
         if (timing) HIPCHK(hipEventRecord(c->ev_a0, s));
         for (int b = pl.nb - 1; b >= 0; --b) {
@@ -529,6 +564,7 @@ int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_
                 a.fallback_list = c->list_fb.as<uint32_t>();
                 a.fallback_count = ctrl + CTRL_FALLBACK;
             }
+            CHK(enqueue_bound(c, a, p, ctrl + CTRL_BOUNDQ + b, n, s));
             CHK(launch_pass(a, p, s));
         }
         if (timing) HIPCHK(hipEventRecord(c->ev_a1, s));
@@ -988,7 +1024,7 @@ int ewfa_destroy(ewfa_ctx_t* c) {
     for (hipEvent_t e : {c->ev_in, c->ev_a0, c->ev_a1, c->ev_e1})
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
-    for (DevBuf* b : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->runs, &c->list_fb,
+    for (DevBuf* b : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->runs, &c->list_fb, &c->bound,
                       &c->h_words, &c->h_pairs, &c->h_results, &c->h_cigars, &c->dense, &c->scan})
         b->release();
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
@@ -1127,7 +1163,9 @@ int ewfa_align_device(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t*
         if (c->use_graph) {
             // with timing: two graphs (align, emit) with events between them
             const int nparts = timing ? 2 : 1;
-            if (!(c->graph && c->graph_key == key && c->graph_policy_gen == c->policy_gen)) {
+            const uint64_t gen = g_buffer_gen.load(std::memory_order_relaxed);
+            if (!(c->graph && c->graph_key == key && c->graph_policy_gen == c->policy_gen &&
+                  c->graph_buffer_gen == gen)) {
                 for (hipGraphExec_t* ge : {&c->graph, &c->graph2})
                     if (*ge) {
                         (void)hipGraphExecDestroy(*ge);
@@ -1155,6 +1193,7 @@ int ewfa_align_device(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t*
                 }
                 c->graph_key = key;
                 c->graph_policy_gen = c->policy_gen;
+                c->graph_buffer_gen = gen;
             }
             if (timing) {
                 HIPCHK(hipEventRecord(c->ev_a0, s));

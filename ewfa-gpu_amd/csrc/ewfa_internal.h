@@ -79,7 +79,13 @@ struct AlignArgs {
     int8_t* cigars;
     uint32_t* fallback_list;
     uint32_t* fallback_count;
+    // per-pair upper bound on the distance (BOUND_NONE: none), written by
+    // bound_kernel and read by the block kernel, which then leaves out the
+    // cells no path of that cost can use; null: no bounds
+    uint32_t* bound;
 };
+
+constexpr uint32_t BOUND_NONE = 0xFFFFFFFFu;
 
 // Bytes of a wave's run scratch for levels 0..dmax: level d's 2d+1 diagonals
 // at d*d, so a pair at distance D touches (D+1)^2 bytes; slots are rounded
@@ -152,6 +158,11 @@ __host__ __device__ inline uint32_t written_cigar_bytes(const ewfa_result_t& r, 
 constexpr uint32_t COMPACT_BLOCK = 1024;
 
 // Kernel launchers (kernels.hip).  All asynchronous on `stream`.
+// Distance bounds of list items from a 64-diagonal band: one wave per pair,
+// the sequences staged in `lds` bytes (lds_bytes_bound), a.queue as its
+// work counter.
+hipError_t launch_bound(const AlignArgs& a, uint32_t grid, size_t lds, hipStream_t stream);
+inline size_t lds_bytes_bound(int seq_words) { return 16u * (size_t)seq_words; }
 hipError_t launch_align(const AlignArgs& a, int waves, bool persistent, uint32_t grid,
                         size_t lds, hipStream_t stream);
 hipError_t launch_align_reg(const AlignArgs& a, int nc, bool persistent, uint32_t grid, size_t lds,

@@ -168,6 +168,18 @@ __device__ __forceinline__ int wave_incl_scan_i32(int x) {
     return x;
 }
 
+// Maximum of x over the 64 lanes (every lane takes part; x >= -1), by the
+// same DPP steps, read from lane 63.
+__device__ __forceinline__ int wave_max_i32(int x) {
+    x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x111, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x112, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x114, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x118, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x142, 0xA, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x143, 0xC, 0xF, false));
+    return __builtin_amdgcn_readlane(x, 63);
+}
+
 // Stream bytes of a match run of r < 255 bases: floor(r/127) bytes of -127
 // then -(r%127) when nonzero (src/kernels.cu:436-446).
 __device__ __forceinline__ int short_run_bytes(int r) {
@@ -438,7 +450,83 @@ __device__ __forceinline__ int extend_le3(const uint32_t* P, const uint32_t* T, 
     return acc;
 }
 
-template <int G, bool BT_LDS>
+// One level of the block kernel: the group's waves take chunks of 64
+// flattened diagonals in turn (chunk c of level d: diagonals 64c - d ..
+// 64c + 63 - d) over the indices ilo..ihi.  Without a bound (BOUNDED false)
+// the range is the whole level 0..2d, whose every chunk starts with a lane
+// inside it.
+template <int G, bool BT_LDS, bool BOUNDED>
+__device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, int lane, const int32_t* cur,
+                                            int32_t* nxt, const uint32_t* P32, const uint32_t* T32, int plen,
+                                            int tlen, uint32_t base, uint64_t* BT, __amdgpu_buffer_rsrc_t BTs) {
+    const int32_t* curm1 = cur - 1;
+    for (int c = (BOUNDED ? ilo >> 6 : 0) + wave; c <= (ihi >> 6); c += G) {
+        int k2;
+        bool in;
+        if constexpr (BOUNDED) {
+            const int t = (c << 6) + lane - ilo;
+            in = (unsigned)t <= (unsigned)(ihi - ilo);
+            k2 = 2 * (t + ilo - d);
+        } else {
+            const int idx2 = (c << 7) + 2 * lane;  // twice the diagonal index 0..2d
+            in = idx2 <= 4 * d;
+            k2 = idx2 - 2 * d;
+        }
+        if (in) {
+            // k2: twice the diagonal.  Candidates from level d-1
+            // (src/kernels.cu:131-138): DEL o(k+1), SUB o(k)+1, INS o(k-1)+1,
+            // doubled; the larger wins, DEL on ties, then SUB
+            // (src/kernels.cu:141-152)
+            const int32_t* q = curm1 + (k2 >> 1);  // diagonals k-1, k, k+1
+            const int ins = q[0], sub = q[1], del = q[2];
+            const int si = max(sub, ins) + 2;
+            const int val = max(del, si);
+            // op ballots: DEL won; SUB at least INS (the op is DEL when its
+            // DEL bit is set, else SUB when this bit is, else INS)
+            const uint64_t lo = __builtin_amdgcn_ballot_w64(del >= si);
+            const uint64_t hi = __builtin_amdgcn_ballot_w64(sub >= ins);
+            const int v = val - k2;
+            // matched bases, capped at the bases left (none past an end); a
+            // valid offset is never left of either start, and a read past an
+            // end stays inside the slot
+            const int mc = min(2 * plen + k2, 2 * tlen);
+            int acc = med3_i32(eq2_le3(P32, T32, v, val), mc - val, 0);
+            if (acc == 64) {
+                int e;
+                do {
+                    e = min(eq2_le3(P32, T32, v + acc, val + acc), mc - val - acc);
+                    acc += e;
+                } while (e == 64);
+            }
+            // nxt[k]: k2 is even, so 2*k2 is the byte offset
+            *reinterpret_cast<int32_t*>(reinterpret_cast<char*>(nxt) + 2 * k2) = val + acc;
+            // the chunk's first lane in the range stores its ballots: lane 0,
+            // unless a bounded range starts inside the chunk
+            bool first;
+            if constexpr (BOUNDED) {
+                const uint64_t ex = __builtin_amdgcn_read_exec();
+                first = __builtin_amdgcn_inverse_ballot_w64(ex & (0ull - ex));
+            } else {
+                first = lane == 0;
+            }
+            if (first) {
+                if constexpr (BT_LDS) {
+                    uint64_t* bq = BT + 2 * (size_t)(base + (uint32_t)c);
+                    bq[0] = lo;
+                    bq[1] = hi;
+                } else {
+                    const u32x4 w = {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
+                    __builtin_amdgcn_raw_buffer_store_b128(w, BTs, 0, (int)(16u * (base + (uint32_t)c)), 0);
+                }
+            }
+        }
+    }
+}
+
+// BOUNDED: the pass reads a distance bound per pair (a.bound) and leaves out
+// the cells beyond it; a separate instantiation, so that passes without
+// bounds keep the registers (and occupancy) of the plain level loop.
+template <int G, bool BT_LDS, bool BOUNDED>
 __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, uint64_t* bt_g,
                            int gtid) {
     const int lane = gtid & 63;
@@ -487,58 +575,31 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
     const int cap = a.max_distance;
     int limit = dcap;
     if (cap > 0 && cap - 1 < limit) limit = cap - 1;
+    // With an upper bound U on the distance, cell (d, k) is computed only when
+    // d + |k - tk| <= U: a path through any other cell costs more than U to
+    // reach the target diagonal, and the cells kept read only cells kept, so
+    // the values the search and its traceback use are unchanged.
+    const uint32_t ub = BOUNDED ? a.bound[pidx] : BOUND_NONE;
+    const bool pruned = BOUNDED && ub <= (uint32_t)limit;
+    const int U = (int)ub;
     bool done = (tk == 0) && (X[0] == 2 * tlen);
     int d = 0;
     int32_t* cur = X;
     int32_t* nxt = Y;
-    const int lane2 = 2 * lane;
     uint32_t base = 0;  // bt_level_base(d): level d's first chunk
-    while (!done && d < limit) {
+    // a search with a bound ends by level U (at most limit); the ranges below
+    // are not empty up to there
+    const int dstop = pruned ? U : limit;
+    while (!done && d < dstop) {
         ++d;
-        const int nch = (d >> 5) + 1;
-        const int32_t* curm1 = cur - 1;
-        for (int c = wave; c < nch; c += G) {
-            const int idx2 = (c << 7) + lane2;  // twice the diagonal index 0..2d
-            // lane 0 of every chunk up to nch-1 is inside the level
-            if (idx2 <= 4 * d) {
-                const int k2 = idx2 - 2 * d;  // twice the diagonal
-                // candidates from level d-1 (src/kernels.cu:131-138): DEL
-                // o(k+1), SUB o(k)+1, INS o(k-1)+1, doubled; the larger wins,
-                // DEL on ties, then SUB (src/kernels.cu:141-152)
-                const int32_t* q = curm1 + (k2 >> 1);  // diagonals k-1, k, k+1
-                const int ins = q[0], sub = q[1], del = q[2];
-                const int si = max(sub, ins) + 2;
-                const int val = max(del, si);
-                // op ballots: DEL won; SUB at least INS (the op is DEL when
-                // its DEL bit is set, else SUB when this bit is, else INS)
-                const uint64_t lo = __builtin_amdgcn_ballot_w64(del >= si);
-                const uint64_t hi = __builtin_amdgcn_ballot_w64(sub >= ins);
-                const int v = val - k2;
-                // matched bases, capped at the bases left (none past an end);
-                // a valid offset is never left of either start, and a read
-                // past an end stays inside the slot
-                const int mc = min(2 * plen + k2, 2 * tlen);
-                int acc = med3_i32(eq2_le3(P32, T32, v, val), mc - val, 0);
-                if (acc == 64) {
-                    int e;
-                    do {
-                        e = min(eq2_le3(P32, T32, v + acc, val + acc), mc - val - acc);
-                        acc += e;
-                    } while (e == 64);
-                }
-                // nxt[k]: k2 is even, so 2*k2 is the byte offset
-                *reinterpret_cast<int32_t*>(reinterpret_cast<char*>(nxt) + 2 * k2) = val + acc;
-                if (lane == 0) {
-                    if constexpr (BT_LDS) {
-                        uint64_t* bq = BT + 2 * (size_t)(base + (uint32_t)c);
-                        bq[0] = lo;
-                        bq[1] = hi;
-                    } else {
-                        const u32x4 w = {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
-                        __builtin_amdgcn_raw_buffer_store_b128(w, BTs, 0, (int)(16u * (base + (uint32_t)c)), 0);
-                    }
-                }
-            }
+        if (BOUNDED && pruned) {
+            // flattened diagonal indices (diagonal k at k + d) of the level's
+            // cells within the bound
+            const int ilo = max(0, tk + d - (U - d));
+            const int ihi = min(2 * d, tk + d + (U - d));
+            block_level<G, BT_LDS, true>(d, ilo, ihi, wave, lane, cur, nxt, P32, T32, plen, tlen, base, BT, BTs);
+        } else {
+            block_level<G, BT_LDS, false>(d, 0, 2 * d, wave, lane, cur, nxt, P32, T32, plen, tlen, base, BT, BTs);
         }
         // sentinels for the next level's reads at |k| = d+1, d+2
         if (gtid < 4) {
@@ -546,7 +607,7 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
             nxt[e] = -2;
         }
         group_sync<G>();
-        base += (uint32_t)nch;
+        base += (uint32_t)((d >> 5) + 1);
         done = (tka <= d) && (nxt[tk] == 2 * tlen);
         int32_t* tmp = cur;
         cur = nxt;
@@ -1120,7 +1181,7 @@ __global__ __launch_bounds__(64, NC <= 3 ? 8 : 6) void align_reg_kernel(AlignArg
 // ---------------------------------------------------------------------------
 // align kernel: G waves per pair, one pair slot per block
 
-template <int G, bool BT_LDS>
+template <int G, bool BT_LDS, bool BOUNDED = false>
 __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persistent) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t s_base;
@@ -1129,7 +1190,7 @@ __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persiste
     const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
     if (!persistent) {
         const uint32_t item = blockIdx.x;
-        if (item < count) align_pair<G, BT_LDS>(a, a.list ? a.list[item] : item, lds, btg, tid);
+        if (item < count) align_pair<G, BT_LDS, BOUNDED>(a, a.list ? a.list[item] : item, lds, btg, tid);
         return;
     }
     for (;;) {
@@ -1147,9 +1208,114 @@ __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persiste
         if (base >= count) break;
         const uint32_t end = min(base + a.grab, count);
         for (uint32_t it = base; it < end; ++it) {
-            align_pair<G, BT_LDS>(a, a.list ? a.list[it] : it, lds, btg, tid);
+            align_pair<G, BT_LDS, BOUNDED>(a, a.list ? a.list[it] : it, lds, btg, tid);
             group_sync<G>();  // the slot is reused by the next pair
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// distance bound: the wavefront (src/kernels.cu:111-180) restricted to a band
+// of 64 diagonals that follows the pair's furthest-reaching cell.  Every
+// offset the band keeps is reached by a path inside the band, so the first
+// level at which it holds (plen, tlen) is the cost of a real alignment: an
+// upper bound on the distance, which the block kernel uses to leave out
+// cells.  The band moves by at most one diagonal a level, towards the cell
+// with the most bases consumed (counted up to the ends), which it looks for
+// every BAND_LOOK levels and otherwise follows by its own shifts.
+
+constexpr int BAND_NONE_OFF = -(1 << 28);  // doubled offset of a diagonal with none
+constexpr int BAND_LOOK = 8;
+
+__device__ __forceinline__ uint32_t band_bound(const AlignArgs& a, uint32_t pidx, uint64_t* slot, int lane) {
+    const ewfa_pair_t pr = a.pairs[pidx];
+    const int plen = (int)pr.plen, tlen = (int)pr.tlen;
+    const int sw = a.seq_words;
+    if (pr.plen > a.max_len || pr.tlen > a.max_len || ((plen + 31) >> 5) + 2 > sw || ((tlen + 31) >> 5) + 2 > sw)
+        return BOUND_NONE;
+    uint64_t* P = slot;
+    uint64_t* T = slot + sw;
+    stage_plain(P, a.words + pr.pattern_off, plen, lane);
+    stage_plain(T, a.words + pr.text_off, tlen, lane);
+    group_sync<1>();
+    const uint32_t* P32 = reinterpret_cast<const uint32_t*>(P);
+    const uint32_t* T32 = reinterpret_cast<const uint32_t*>(T);
+    const int tk = tlen - plen;
+    const int tka = tk < 0 ? -tk : tk;
+    const int o00 = extend_le3(P32, T32, plen, tlen, 0, 0);
+    if (tk == 0 && o00 == tlen) return 0u;
+    int limit = a.dcap;
+    const int cap = a.max_distance;
+    if (cap > 0 && cap - 1 < limit) limit = cap - 1;
+    const int pl2 = 2 * plen, tl2 = 2 * tlen;
+    // lane j holds diagonal base + j, its offset doubled
+    int base = -32;
+    int o = lane == 32 ? 2 * o00 : BAND_NONE_OFF;
+    int best = 32;
+    for (int d = 1; d <= limit; ++d) {
+        const int s = best > 33 ? 1 : best < 31 ? -1 : 0;
+        base += s;
+        best -= s;
+        // diagonals k-1, k, k+1 of level d-1 sit in lanes j+s-1, j+s, j+s+1;
+        // lanes shifted in from outside the band read none
+        int ins, sub, del;
+        if (s == 0) {
+            ins = __builtin_amdgcn_update_dpp(BAND_NONE_OFF, o, 0x138, 0xF, 0xF, false);  // wave_shr:1
+            sub = o;
+            del = __builtin_amdgcn_update_dpp(BAND_NONE_OFF, o, 0x130, 0xF, 0xF, false);  // wave_shl:1
+        } else if (s > 0) {
+            ins = o;
+            sub = __builtin_amdgcn_update_dpp(BAND_NONE_OFF, o, 0x130, 0xF, 0xF, false);
+            del = __builtin_amdgcn_update_dpp(BAND_NONE_OFF, sub, 0x130, 0xF, 0xF, false);
+        } else {
+            del = o;
+            sub = __builtin_amdgcn_update_dpp(BAND_NONE_OFF, o, 0x138, 0xF, 0xF, false);
+            ins = __builtin_amdgcn_update_dpp(BAND_NONE_OFF, sub, 0x138, 0xF, 0xF, false);
+        }
+        const int val = max(del, max(sub, ins) + 2);
+        const int k2 = 2 * (base + lane);
+        int acc = 0;
+        if (val >= 0) {
+            // a real path's offset is never left of either start; the first
+            // read is clamped to the ends, past which nothing is matched
+            const int v2 = val - k2;
+            const int left = min(pl2 - v2, tl2 - val);
+            acc = med3_i32(eq2_le3(P32, T32, min(v2, pl2), min(val, tl2)), left, 0);
+            if (acc == 64) {
+                int e;
+                do {
+                    e = min(eq2_le3(P32, T32, v2 + acc, val + acc), left - acc);
+                    acc += e;
+                } while (e == 64);
+            }
+        }
+        o = val >= 0 ? val + acc : BAND_NONE_OFF;
+        const int tj = tk - base;
+        if (tka <= d && tj >= 0 && tj < 64 && __builtin_amdgcn_readlane(o, tj) == tl2) return (uint32_t)d;
+        if (d % BAND_LOOK == 0) {
+            const int key = o >= 0 ? ((min(o - k2, pl2) + min(o, tl2)) << 6) | lane : -1;
+            best = wave_max_i32(key) & 63;
+        }
+    }
+    return BOUND_NONE;
+}
+
+// One wave per pair from a work counter (a.queue) over the list; writes
+// a.bound[pair] (BOUND_NONE when the band does not reach the end within the
+// pass's level limit).
+__global__ __launch_bounds__(64) void bound_kernel(AlignArgs a) {
+    extern __shared__ uint64_t lds[];
+    const int lane = threadIdx.x;
+    const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
+    for (;;) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(a.queue, 1u);
+        const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+        if (item >= count) break;
+        const uint32_t pidx = a.list ? a.list[item] : item;
+        const uint32_t u = band_bound(a, pidx, lds, lane);
+        if (lane == 0) a.bound[pidx] = u;
+        group_sync<1>();  // the slot is staged again for the next pair
     }
 }
 
@@ -1682,8 +1848,12 @@ __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, uint32_t val
 template <int G, bool BT_LDS>
 static hipError_t launch_align_t(const AlignArgs& a, bool persistent, uint32_t grid, size_t lds,
                                  hipStream_t stream) {
-    hipLaunchKernelGGL((align_kernel<G, BT_LDS>), dim3(grid), dim3(64 * G), lds, stream, a,
-                       persistent ? 1 : 0);
+    if (a.bound && !BT_LDS)
+        hipLaunchKernelGGL((align_kernel<G, false, true>), dim3(grid), dim3(64 * G), lds, stream, a,
+                           persistent ? 1 : 0);
+    else
+        hipLaunchKernelGGL((align_kernel<G, BT_LDS>), dim3(grid), dim3(64 * G), lds, stream, a,
+                           persistent ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -1704,6 +1874,12 @@ hipError_t launch_align(const AlignArgs& a, int waves, bool persistent, uint32_t
                           : launch_align_t<16, false>(a, persistent, grid, lds, stream);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_bound(const AlignArgs& a, uint32_t grid, size_t lds, hipStream_t stream) {
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(bound_kernel, dim3(grid), dim3(64), lds, stream, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_align_reg(const AlignArgs& a, int nc, bool persistent, uint32_t grid, size_t lds,
@@ -1734,25 +1910,27 @@ int align_reg_blocks_per_cu(int nc, size_t lds) {
     return blocks;
 }
 
-template <int G, bool BT_LDS>
+template <int G, bool BT_LDS, bool BOUNDED = false>
 static const void* align_fn() {
-    return reinterpret_cast<const void*>(&align_kernel<G, BT_LDS>);
+    return reinterpret_cast<const void*>(&align_kernel<G, BT_LDS, BOUNDED>);
 }
 
-static const void* align_fn_for(int waves, bool bt_lds) {
+// variant 0: backtrace in HBM, 1: in LDS, 2: in HBM with distance bounds
+static const void* align_fn_for(int waves, int variant) {
     switch (waves) {
-        case 1: return bt_lds ? align_fn<1, true>() : align_fn<1, false>();
-        case 2: return bt_lds ? align_fn<2, true>() : align_fn<2, false>();
-        case 4: return bt_lds ? align_fn<4, true>() : align_fn<4, false>();
-        case 8: return bt_lds ? align_fn<8, true>() : align_fn<8, false>();
-        case 16: return bt_lds ? align_fn<16, true>() : align_fn<16, false>();
+        case 1: return variant == 1 ? align_fn<1, true>() : variant ? align_fn<1, false, true>() : align_fn<1, false>();
+        case 2: return variant == 1 ? align_fn<2, true>() : variant ? align_fn<2, false, true>() : align_fn<2, false>();
+        case 4: return variant == 1 ? align_fn<4, true>() : variant ? align_fn<4, false, true>() : align_fn<4, false>();
+        case 8: return variant == 1 ? align_fn<8, true>() : variant ? align_fn<8, false, true>() : align_fn<8, false>();
+        case 16:
+            return variant == 1 ? align_fn<16, true>() : variant ? align_fn<16, false, true>() : align_fn<16, false>();
         default: return nullptr;
     }
 }
 
 hipError_t align_set_max_lds(int waves, size_t lds) {
-    for (int b = 0; b < 2; ++b) {
-        const void* f = align_fn_for(waves, b != 0);
+    for (int b = 0; b < 3; ++b) {
+        const void* f = align_fn_for(waves, b);
         if (!f) return hipErrorInvalidValue;
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -1761,7 +1939,7 @@ hipError_t align_set_max_lds(int waves, size_t lds) {
 }
 
 int align_blocks_per_cu(int waves, size_t lds) {
-    const void* f = align_fn_for(waves, true);
+    const void* f = align_fn_for(waves, 1);
     if (!f) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, 64 * waves, lds) != hipSuccess)

@@ -1,0 +1,81 @@
+"""Block passes that leave out cells beyond a distance bound.
+
+A block pass with a budget of 2,048 levels or more first bounds each pair's
+distance with a 64-diagonal band that follows the furthest-reaching cell, and
+then computes only the cells (d, k) with d + |k - tk| <= bound.  The results
+must be those of the full wavefront (src/kernels.cu:111-180, the tie-break at
+:141-152), so every case here is checked against the oracle bit for bit:
+long pairs with indel runs (the band must follow the path's jumps), pairs
+whose distance exceeds the reference cap (the band fails and the pass runs
+unbounded), low-complexity pairs full of equal-cost paths, and each wave
+count of the block kernel (a level's first computed chunk can start inside
+a chunk, where lane 0 is outside the range).
+"""
+import numpy as np
+import pytest
+
+import ewfa
+import oracle_py as orc
+from test_gpu_stress import oracle_streams, stress_pairs
+
+pytestmark = pytest.mark.gpu
+
+BOUNDED = {
+    "block1": "4000000000 1 2048 0 1\n",
+    "block4": "4000000000 4 2048 0 1\n",
+    "block16": "4000000000 16 2048 0 1\n",
+}
+
+
+def check(res, pairs, cig, d, st, label, capped_ok=False):
+    streams = ewfa.streams_from(res, pairs, cig)
+    ok_status = (ewfa.PAIR_OK, ewfa.PAIR_CAPPED, ewfa.PAIR_UNDEFINED) if capped_ok else (ewfa.PAIR_OK,)
+    for i in range(len(res)):
+        good = (int(res[i]["status"]) in ok_status and int(res[i]["distance"]) == int(d[i])
+                and np.array_equal(streams[i], st[i]))
+        assert good, (label, i, int(res[i]["status"]), int(res[i]["distance"]), int(d[i]),
+                      streams[i][:32].tolist(), st[i][:32].tolist())
+
+
+def policy_file(tmp_path, name, text):
+    path = str(tmp_path / f"{name}.policy")
+    with open(path, "w") as f:
+        f.write(text)
+    return path
+
+
+@pytest.mark.parametrize("name", sorted(BOUNDED))
+def test_long_pairs_with_indel_runs(tmp_path, name):
+    # 2,000..5,000 bp at 10-15% edits, 30% of them as indel runs: distances
+    # of about 200..700 levels
+    print(f"case bounded {name}: 120 pairs 2000..5000 bp 10-15% with indel runs", flush=True)
+    w = ewfa.Workload(ewfa.synth_spec(kind=2, seed=41, len_lo=2000, len_hi=5000, err_lo=0.10, err_hi=0.15), 120)
+    ok, d, sl, st = orc.batch_workload(w)
+    assert ok == w.n
+    with ewfa.Context(policy_path=policy_file(tmp_path, name, BOUNDED[name])) as ctx:
+        res, cig = ctx.align_host(w.words, w.pairs, w.n, w.cigar_bytes)
+    check(res, w.pairs, cig, d, st, name)
+
+
+def test_low_complexity_pairs(tmp_path):
+    P, T = stress_pairs(303, 600)
+    print(f"case bounded low-complexity {len(P)} pairs", flush=True)
+    d, st = oracle_streams(P, T)
+    words, pairs, cb, ml = ewfa.from_sequences(P, T)
+    for name in ("block4", "block16"):
+        with ewfa.Context(policy_path=policy_file(tmp_path, name, BOUNDED[name])) as ctx:
+            res, cig = ctx.align_host(words, pairs, len(P), cb)
+        check(res, pairs, cig, d, st, name)
+
+
+def test_reference_cap(tmp_path):
+    # with the cap of 64 the band searches 63 levels: pairs beyond it get no
+    # bound and keep the capped walk from level 63
+    print("case bounded cap 64: 300 pairs 300..3000 bp 2-10%", flush=True)
+    w = ewfa.Workload(ewfa.synth_spec(kind=1, seed=42, len_lo=300, len_hi=3000, err_lo=0.02, err_hi=0.10), 300)
+    ok, d, sl, st = orc.batch_workload(w, max_distance=64)  # ok counts the uncapped pairs
+    assert 0 < ok < w.n
+    with ewfa.Context(max_distance=64, policy_path=policy_file(tmp_path, "block4", BOUNDED["block4"])) as ctx:
+        res, cig = ctx.align_host(w.words, w.pairs, w.n, w.cigar_bytes)
+    check(res, w.pairs, cig, d, st, "cap64", capped_ok=True)
+    assert any(int(r["status"]) == ewfa.PAIR_CAPPED for r in res)
