@@ -7,9 +7,10 @@ must be those of the full wavefront (src/kernels.cu:111-180, the tie-break at
 :141-152), so every case here is checked against the oracle bit for bit:
 long pairs with indel runs (the band must follow the path's jumps), pairs
 whose distance exceeds the reference cap (the band fails and the pass runs
-unbounded), low-complexity pairs full of equal-cost paths, and each wave
-count of the block kernel (a level's first computed chunk can start inside
-a chunk, where lane 0 is outside the range).
+unbounded), low-complexity pairs full of equal-cost paths, each wave count
+of the block kernel (a level's first computed chunk can start inside a
+chunk, where lane 0 is outside the range), and distances above a bounded
+pass's budget (those pairs finish in the deferral passes, bounded again).
 """
 import numpy as np
 import pytest
@@ -79,3 +80,17 @@ def test_reference_cap(tmp_path):
         res, cig = ctx.align_host(w.words, w.pairs, w.n, w.cigar_bytes)
     check(res, w.pairs, cig, d, st, "cap64", capped_ok=True)
     assert any(int(r["status"]) == ewfa.PAIR_CAPPED for r in res)
+
+
+def test_distances_above_the_budget(tmp_path):
+    # budget 2,048 on 16 waves; pairs of 8-10 kbp at 25-35% edits have
+    # distances of about 2,000-2,700, so most are deferred and finish in
+    # the deferral passes
+    print("case bounded beyond budget: 16 pairs 8000..10000 bp 25-35%", flush=True)
+    w = ewfa.Workload(ewfa.synth_spec(kind=1, seed=43, len_lo=8000, len_hi=10000, err_lo=0.25, err_hi=0.35), 16)
+    ok, d, sl, st = orc.batch_workload(w)
+    assert ok == w.n
+    assert (d > 2048).any() and (d <= 2048).any()
+    with ewfa.Context(policy_path=policy_file(tmp_path, "block16", BOUNDED["block16"])) as ctx:
+        res, cig = ctx.align_host(w.words, w.pairs, w.n, w.cigar_bytes)
+    check(res, w.pairs, cig, d, st, "beyond-budget")
