@@ -436,6 +436,17 @@ __device__ __forceinline__ int eq2_le3(const uint32_t* P, const uint32_t* T, int
     return (int)(z & ~1u);  // base b of the window sits in bits 2b, 2b+1
 }
 
+// The same over a 16-base window (two words per sequence): twice the equal
+// leading bases, 0..32.
+__device__ __forceinline__ int eq2_le2(const uint32_t* P, const uint32_t* T, int v2, int h2) {
+    const uint32_t* p = P + (v2 >> 5);
+    const uint32_t* t = T + (h2 >> 5);
+    const uint32_t x = __builtin_amdgcn_alignbit(p[1], p[0], (uint32_t)v2) ^
+                       __builtin_amdgcn_alignbit(t[1], t[0], (uint32_t)h2);
+    const uint32_t z = x ? (uint32_t)__builtin_ctz(x) : 32u;
+    return (int)(z & ~1u);
+}
+
 // Equal bases from (v, h) up to the first mismatch or either end.  Positions
 // past an end extend by nothing; the reads are clamped into the copies.
 __device__ __forceinline__ int extend_le3(const uint32_t* P, const uint32_t* T, int plen, int tlen, int v, int h) {
@@ -488,10 +499,11 @@ __device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, i
             const int v = val - k2;
             // matched bases, capped at the bases left (none past an end); a
             // valid offset is never left of either start, and a read past an
-            // end stays inside the slot
+            // end stays inside the slot.  The first step compares 16 bases
+            // (most cells' runs end there), the rest 32 at a time.
             const int mc = min(2 * plen + k2, 2 * tlen);
-            int acc = med3_i32(eq2_le3(P32, T32, v, val), mc - val, 0);
-            if (acc == 64) {
+            int acc = med3_i32(eq2_le2(P32, T32, v, val), mc - val, 0);
+            if (acc == 32) {
                 int e;
                 do {
                     e = min(eq2_le3(P32, T32, v + acc, val + acc), mc - val - acc);
