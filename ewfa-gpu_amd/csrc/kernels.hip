@@ -479,9 +479,8 @@ __device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, i
             in = (unsigned)t <= (unsigned)(ihi - ilo);
             k2 = 2 * (t + ilo - d);
         } else {
-            const int idx2 = (c << 7) + 2 * lane;  // twice the diagonal index 0..2d
-            in = idx2 <= 4 * d;
-            k2 = idx2 - 2 * d;
+            k2 = 2 * lane + ((c << 7) - 2 * d);  // twice the diagonal 64c + lane - d
+            in = k2 <= 2 * d;
         }
         if (in) {
             // k2: twice the diagonal.  Candidates from level d-1
