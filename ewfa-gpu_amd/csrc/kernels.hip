@@ -1447,20 +1447,15 @@ __device__ __forceinline__ void emit_one(const EmitArgs& a, uint32_t i) {
             off += e;
             if ((s & 31) == 0) word = ow[s >> 5];
             const int op = (int)((word >> ((s & 31) << 1)) & 3ull);
-            if (op == OP_DEL) {
-                w.put('D');
-                --k;
-            } else if (op == OP_SUB) {
-                w.put('X');
-                ++off;
-            } else if (op == OP_INS) {
-                w.put('I');
-                ++k;
-                ++off;
-            } else {
+            if (op == 0) {
                 stopped = true;  // op 0 ends the walk (src/kernels.cu:471-478)
                 break;
             }
+            // one byte and two selects for the three ops, so that the lanes
+            // of a wave, each at its own op, do not split three ways
+            w.put(op == OP_DEL ? 'D' : op == OP_SUB ? 'X' : 'I');
+            k += (op == OP_INS) - (op == OP_DEL);
+            off += op != OP_DEL;
         }
         if (!stopped) w.run(extend_seq(P, T, off - k, off));
     }
@@ -1566,20 +1561,13 @@ __global__ __launch_bounds__(64) void emit_lds_kernel(EmitArgs a, int sw) {
             off += e;
             if ((s & 31) == 0) word = ow[s >> 5];
             const int op = (int)((word >> ((s & 31) << 1)) & 3ull);
-            if (op == OP_DEL) {
-                w.put('D');
-                --k;
-            } else if (op == OP_SUB) {
-                w.put('X');
-                ++off;
-            } else if (op == OP_INS) {
-                w.put('I');
-                ++k;
-                ++off;
-            } else {
+            if (op == 0) {
                 stopped = true;  // op 0 ends the walk (src/kernels.cu:471-478)
                 break;
             }
+            w.put(op == OP_DEL ? 'D' : op == OP_SUB ? 'X' : 'I');  // as the global walk
+            k += (op == OP_INS) - (op == OP_DEL);
+            off += op != OP_DEL;
         }
         if (!stopped) w.run(extend_w(P, T, plen, tlen, off - k, off));
     }
