@@ -430,12 +430,20 @@ void fill_pass_args(ewfa::AlignArgs& a, const Pass& p, ewfa_ctx* c) {
 // pairs, paths with long runs): one wave per pair with its sequences in LDS
 // when they fit, else one lane per pair.  For whole buckets one lane per pair
 // keeps more pairs in flight, which wins on long sequences.
+// Pairs per wave of the one-lane-per-pair emit: EMIT_PER_WAVE when the
+// launch fills every CU with at least 16 such waves, fewer (down to one) for
+// smaller launches, so that few long pairs still get many waves.
+uint32_t emit_per_wave(const ewfa_ctx* c, uint32_t items) {
+    const uint32_t fill = (uint32_t)c->num_cus * 16u;
+    return std::min<uint32_t>(std::max<uint32_t>(items / fill, 1u), (uint32_t)ewfa::EMIT_PER_WAVE);
+}
+
 int emit_list(ewfa_ctx* c, const ewfa::EmitArgs& e, uint32_t items, uint32_t max_len, hipStream_t s) {
     const int ow = overlapped_words_for(max_len);
     if ((size_t)2 * ow * 8 <= ewfa::EMIT_WAVE_MAX_LDS)
         HIPCHK(ewfa::launch_emit_wave(e, items, ow, (uint32_t)c->num_cus * 8u, s));
     else
-        HIPCHK(ewfa::launch_emit(e, items, s));
+        HIPCHK(ewfa::launch_emit(e, items, emit_per_wave(c, items), s));
     return EWFA_OK;
 }
 
@@ -444,7 +452,7 @@ int emit(ewfa_ctx* c, const ewfa::EmitArgs& e, uint32_t items, uint32_t max_len,
     if (staged && sw <= ewfa::EMIT_LDS_MAX_SW)
         HIPCHK(ewfa::launch_emit_lds(e, items, sw, s));
     else
-        HIPCHK(ewfa::launch_emit(e, items, s));
+        HIPCHK(ewfa::launch_emit(e, items, emit_per_wave(c, items), s));
     return EWFA_OK;
 }
 

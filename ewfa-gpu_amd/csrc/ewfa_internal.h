@@ -104,6 +104,7 @@ struct EmitArgs {
     const uint32_t* list;
     uint32_t count;
     const uint32_t* count_ptr;
+    uint32_t per_wave;  // emit_kernel: pairs per wave (set by launch_emit)
 };
 
 // LDS bytes one pair slot needs for the given budget.
@@ -168,12 +169,13 @@ hipError_t launch_align(const AlignArgs& a, int waves, bool persistent, uint32_t
 hipError_t launch_align_reg(const AlignArgs& a, int nc, bool persistent, uint32_t grid, size_t lds,
                             hipStream_t stream);
 int align_reg_blocks_per_cu(int nc, size_t lds);
-// One lane per pair, grid-stride: grid_items bounds the launch (the count may
-// be read on the device).
-hipError_t launch_emit(const EmitArgs& a, uint32_t grid_items, hipStream_t stream);
-// Pairs per wave of the one-lane-per-pair emit: a quarter of the lanes, so
-// four times the waves are in flight to hide the walks' memory latency (1.6%
-// faster on the mixed-length workload than a full wave).
+// One lane per pair, per_wave pairs per wave (1..64), grid-stride:
+// grid_items bounds the launch (the count may be read on the device).
+hipError_t launch_emit(const EmitArgs& a, uint32_t grid_items, uint32_t per_wave, hipStream_t stream);
+// Pairs per wave of the one-lane-per-pair emit for large launches: a quarter
+// of the lanes, so four times the waves are in flight to hide the walks'
+// memory latency (1.6% faster on the mixed-length workload than a full wave).
+// Smaller launches spread their pairs thinner, down to one per wave.
 constexpr int EMIT_PER_WAVE = 16;
 // Emit with one wave per pair and its sequences staged in LDS as overlapped
 // copies of sw words (2 * sw * 8 bytes per block, at most EMIT_WAVE_MAX_LDS);

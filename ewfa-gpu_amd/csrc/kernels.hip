@@ -1369,15 +1369,16 @@ __device__ __forceinline__ void emit_one(const EmitArgs& a, uint32_t i);
 
 // Grid-stride over the items, so a launch sized for the largest count can
 // run over a count that is only known on the device.
-// One lane per pair, EMIT_PER_WAVE pairs per wave (lanes 0..EMIT_PER_WAVE-1),
+// One lane per pair, a.per_wave pairs per wave (lanes 0..per_wave-1),
 // grid-stride over the items.
 __global__ __launch_bounds__(256) void emit_kernel(EmitArgs a) {
     const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
     const uint32_t lane = threadIdx.x & 63u;
-    if (lane >= (uint32_t)EMIT_PER_WAVE) return;
+    const uint32_t pw = a.per_wave;
+    if (lane >= pw) return;
     const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
-    const uint32_t stride = gridDim.x * 4u * (uint32_t)EMIT_PER_WAVE;
-    for (uint32_t item = wave * (uint32_t)EMIT_PER_WAVE + lane; item < count; item += stride)
+    const uint32_t stride = gridDim.x * 4u * pw;
+    for (uint32_t item = wave * pw + lane; item < count; item += stride)
         emit_one(a, a.list ? a.list[item] : item);
 }
 
@@ -1946,11 +1947,14 @@ int align_blocks_per_cu(int waves, size_t lds) {
     return blocks;
 }
 
-hipError_t launch_emit(const EmitArgs& a, uint32_t grid_items, hipStream_t stream) {
+hipError_t launch_emit(const EmitArgs& a, uint32_t grid_items, uint32_t per_wave, hipStream_t stream) {
     if (grid_items == 0) return hipSuccess;
-    const uint32_t per_block = 4u * (uint32_t)EMIT_PER_WAVE;
+    if (per_wave < 1 || per_wave > 64) return hipErrorInvalidValue;
+    EmitArgs b = a;
+    b.per_wave = per_wave;
+    const uint32_t per_block = 4u * per_wave;
     const uint32_t blocks = (grid_items + per_block - 1) / per_block;
-    hipLaunchKernelGGL(emit_kernel, dim3(blocks < 16384u ? blocks : 16384u), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(emit_kernel, dim3(blocks < 16384u ? blocks : 16384u), dim3(256), 0, stream, b);
     return hipGetLastError();
 }
 
