@@ -1288,11 +1288,12 @@ __device__ __forceinline__ uint32_t band_bound(const AlignArgs& a, uint32_t pidx
         int acc = 0;
         if (val >= 0) {
             // a real path's offset is never left of either start; the first
-            // read is clamped to the ends, past which nothing is matched
+            // read (16 bases, then 32 at a time) is clamped to the ends, past
+            // which nothing is matched
             const int v2 = val - k2;
             const int left = min(pl2 - v2, tl2 - val);
-            acc = med3_i32(eq2_le3(P32, T32, min(v2, pl2), min(val, tl2)), left, 0);
-            if (acc == 64) {
+            acc = med3_i32(eq2_le2(P32, T32, min(v2, pl2), min(val, tl2)), left, 0);
+            if (acc == 32) {
                 int e;
                 do {
                     e = min(eq2_le3(P32, T32, v2 + acc, val + acc), left - acc);
