@@ -1120,7 +1120,8 @@ int ewfa_context_memory(ewfa_ctx_t* c, size_t* bytes) {
     if (!c || !bytes) return fail(EWFA_ERR_ARG, "null argument");
     size_t b = 0;
     for (const DevBuf* d : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->runs,
-                            &c->list_fb, &c->h_words, &c->h_pairs, &c->h_results, &c->h_cigars, &c->dense, &c->scan})
+                            &c->list_fb, &c->bound, &c->h_words, &c->h_pairs, &c->h_results, &c->h_cigars,
+                            &c->dense, &c->scan})
         b += d->bytes;
     *bytes = b;
     return EWFA_OK;

@@ -175,6 +175,34 @@ def test_device_api_with_library_buffers():
         assert_matches_oracle(res, w.pairs, cig, d, st, f"device-api flags {flags}")
 
 
+def test_graph_replay_after_scratch_grows():
+    """A replayed plan graph must not use scratch buffers that a call in between
+    replaced: a small device call (graph captured), a host call on a larger
+    c3-like input (the op, list, bucket and backtrace scratch grow), then the
+    small call again with the same buffers, sizes and policy."""
+    say("case graph replay after scratch growth: c2 600 pairs, then c3-like 6000 pairs")
+    w = workload_case("c2", 600, first=11)
+    ok, d, sl, st = orc.batch_workload(w)
+    big = ewfa.Workload(ewfa.synth_spec(kind=1, seed=12, len_lo=200, len_hi=6000, err_lo=0.02,
+                                        err_hi=0.10), 6000)
+    dw = ewfa.DeviceBuffer.from_array(w.words)
+    dp = ewfa.DeviceBuffer.from_array(w.pairs[:w.n])
+    dr = ewfa.DeviceBuffer(w.n * 16)
+    dc = ewfa.DeviceBuffer(w.cigar_bytes)
+    with ewfa.Context() as ctx:
+        ctx.align_device(dw.ptr, dp.ptr, w.n, w.max_len, dr.ptr, dc.ptr, None)
+        before = ctx.memory_bytes()
+        ctx.align_host(big.words, big.pairs, big.n, big.cigar_bytes)
+        assert ctx.memory_bytes() > before  # the scratch was replaced by larger buffers
+        dr.fill_zero()
+        dc.fill_zero()
+        ctx.align_device(dw.ptr, dp.ptr, w.n, w.max_len, dr.ptr, dc.ptr, None)
+        ewfa.synchronize()
+        res = dr.download(ewfa.RESULT_DTYPE, w.n)
+        cig = dc.download(np.int8)
+    assert_matches_oracle(res, w.pairs, cig, d, st, "replay after growth")
+
+
 def test_pack_kernel_matches_host():
     say("case pack kernel 500 sequences, stride 304")
     rng = np.random.default_rng(5)
