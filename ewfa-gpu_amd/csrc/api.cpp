@@ -161,6 +161,7 @@ struct Pass {
     int grab;
     int seq_words;
     int reg;  // register-wavefront chunks (0: offsets staged in LDS)
+    int off16;  // 16-bit LDS offsets (block kernel, backtrace in HBM, no bounds)
     size_t lds;
     uint32_t grid;
     bool persistent;
@@ -187,8 +188,20 @@ int max_dcap_for(int seq_words, int bt_lds, size_t budget) {
     return lo;
 }
 
+// Block passes with budgets of at least this many levels take distance
+// bounds (enqueue_bound).
+constexpr int BOUND_MIN_DCAP = 2048;
+
+// 16-bit offsets for a block pass without bounds whose largest doubled
+// offset, 2 * (max_len + dcap) (an offset runs at most one base a level past
+// an end), fits: half the offsets' LDS, more pairs per CU.
+void choose_off16(Pass& p, uint32_t max_len) {
+    p.off16 = (!p.reg && !p.bt_lds && p.dcap < BOUND_MIN_DCAP && (size_t)max_len + (size_t)p.dcap <= 16383) ? 1 : 0;
+}
+
 int finish_pass(ewfa_ctx* c, Pass& p, uint32_t items) {
-    p.lds = p.reg ? ewfa::lds_bytes_reg(p.seq_words, p.dcap, p.reg) : ewfa::lds_bytes(p.seq_words, p.dcap, p.bt_lds);
+    p.lds = p.reg ? ewfa::lds_bytes_reg(p.seq_words, p.dcap, p.reg)
+                  : ewfa::lds_bytes(p.seq_words, p.dcap, p.bt_lds, p.off16);
     if (p.lds > lds_budget(c))
         return fail(EWFA_ERR_RANGE, "pass needs %zu bytes of LDS (device has %zu)", p.lds, c->lds_max);
     if (p.lds > 65536) {
@@ -196,7 +209,8 @@ int finish_pass(ewfa_ctx* c, Pass& p, uint32_t items) {
         HIPCHK(ewfa::align_set_max_lds(p.waves, p.lds));
     }
     if (p.persistent) {
-        int bpc = p.reg ? ewfa::align_reg_blocks_per_cu(p.reg, p.lds) : ewfa::align_blocks_per_cu(p.waves, p.lds);
+        int bpc = p.reg ? ewfa::align_reg_blocks_per_cu(p.reg, p.lds)
+                        : ewfa::align_blocks_per_cu(p.waves, p.lds, p.off16 ? 3 : 1);
         if (bpc < 1) bpc = 1;
         uint64_t g = (uint64_t)bpc * (uint64_t)c->num_cus;
         if (items > 0 && g > items) g = items;
@@ -232,6 +246,7 @@ int pass_from_bucket(ewfa_ctx* c, const ewfa::BucketPolicy& b, uint32_t blen, ui
             p.seq_words = ow;
         }
     }
+    choose_off16(p, blen);
     return finish_pass(c, p, items);
 }
 
@@ -328,6 +343,7 @@ int next_pass(ewfa_ctx* c, uint32_t max_len, int prev_dcap, uint32_t items, Pass
     const int most_lds = max_dcap_for(p.seq_words, 1, c->lds_max / 2);
     p.bt_lds = (most_lds >= want) ? 1 : 0;
     if (want > 2048) p.waves = 8;
+    choose_off16(p, max_len);
     return finish_pass(c, p, items);
 }
 
@@ -385,8 +401,6 @@ int launch_pass(const ewfa::AlignArgs& a, const Pass& p, hipStream_t s) {
 // budgets under BOUND_MIN_DCAP levels (where the measured saving did not pay
 // for the bound), register passes, and sequences whose staging exceeds
 // 64 KiB of LDS go without.  `queue` must hold 0.
-constexpr int BOUND_MIN_DCAP = 2048;
-
 int enqueue_bound(ewfa_ctx* c, ewfa::AlignArgs& a, const Pass& p, uint32_t* queue, uint32_t n, hipStream_t s) {
     a.bound = nullptr;
     const size_t lds = ewfa::lds_bytes_bound(p.seq_words);
@@ -421,6 +435,7 @@ void fill_pass_args(ewfa::AlignArgs& a, const Pass& p, ewfa_ctx* c) {
     a.grab = (uint32_t)p.grab;
     a.bt_global = c->bt.as<uint64_t>();
     a.bt_slot_words = 2u * (uint64_t)ewfa::bt_chunks((uint32_t)p.dcap);
+    a.off16 = p.off16;
 }
 
 // The emit kernel for a pass: sequences staged in LDS when they fit (the

@@ -83,6 +83,8 @@ struct AlignArgs {
     // bound_kernel and read by the block kernel, which then leaves out the
     // cells no path of that cost can use; null: no bounds
     uint32_t* bound;
+    // 16-bit offsets in LDS (the pass's 2 * (max_len + dcap) fits)
+    int32_t off16;
 };
 
 constexpr uint32_t BOUND_NONE = 0xFFFFFFFFu;
@@ -108,9 +110,9 @@ struct EmitArgs {
 };
 
 // LDS bytes one pair slot needs for the given budget.
-inline size_t lds_bytes(int seq_words, int dcap, int bt_lds) {
-    size_t b = 16u * (size_t)seq_words;             // pattern + text
-    b += 8u * (size_t)(2 * dcap + 5);                // two offset generations
+inline size_t lds_bytes(int seq_words, int dcap, int bt_lds, int off16 = 0) {
+    size_t b = 16u * (size_t)seq_words;                   // pattern + text
+    b += (off16 ? 4u : 8u) * (size_t)(2 * dcap + 5);      // two offset generations
     b = (b + 15) & ~(size_t)15;
     if (bt_lds) b += 16u * (size_t)bt_chunks((uint32_t)dcap);
     return b;
@@ -200,7 +202,9 @@ hipError_t launch_bucketize(const ewfa_pair_t* pairs, uint32_t n, const BucketTa
 hipError_t launch_compact(const ewfa_pair_t* pairs, const ewfa_result_t* results, const int8_t* cigars, size_t n,
                           uint64_t* scan, int8_t* dense, hipStream_t stream);
 // Occupancy of the align kernel for a configuration (blocks per CU).
-int align_blocks_per_cu(int waves, size_t lds);
+// Resident blocks per CU of a block-kernel variant (0: backtrace in HBM, 1: in
+// LDS, 2: with distance bounds, 3: with 16-bit offsets).
+int align_blocks_per_cu(int waves, size_t lds, int variant = 1);
 hipError_t align_set_max_lds(int waves, size_t lds);
 
 }  // namespace ewfa

@@ -23,6 +23,8 @@
 //                            (src/kernels.cu:313-394).
 //
 // Integer work only; nothing here touches MFMA.
+#include <type_traits>
+
 #include "ewfa_internal.h"
 
 namespace ewfa {
@@ -466,11 +468,11 @@ __device__ __forceinline__ int extend_le3(const uint32_t* P, const uint32_t* T, 
 // 64c + 63 - d) over the indices ilo..ihi.  Without a bound (BOUNDED false)
 // the range is the whole level 0..2d, whose every chunk starts with a lane
 // inside it.
-template <int G, bool BT_LDS, bool BOUNDED>
-__device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, int lane, const int32_t* cur,
-                                            int32_t* nxt, const uint32_t* P32, const uint32_t* T32, int plen,
+template <int G, bool BT_LDS, bool BOUNDED, typename Off>
+__device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, int lane, const Off* cur,
+                                            Off* nxt, const uint32_t* P32, const uint32_t* T32, int plen,
                                             int tlen, uint32_t base, uint64_t* BT, __amdgpu_buffer_rsrc_t BTs) {
-    const int32_t* curm1 = cur - 1;
+    const Off* curm1 = cur - 1;
     for (int c = (BOUNDED ? ilo >> 6 : 0) + wave; c <= (ihi >> 6); c += G) {
         int k2;
         bool in;
@@ -487,7 +489,7 @@ __device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, i
             // (src/kernels.cu:131-138): DEL o(k+1), SUB o(k)+1, INS o(k-1)+1,
             // doubled; the larger wins, DEL on ties, then SUB
             // (src/kernels.cu:141-152)
-            const int32_t* q = curm1 + (k2 >> 1);  // diagonals k-1, k, k+1
+            const Off* q = curm1 + (k2 >> 1);  // diagonals k-1, k, k+1
             const int ins = q[0], sub = q[1], del = q[2];
             const int si = max(sub, ins) + 2;
             const int val = max(del, si);
@@ -510,7 +512,7 @@ __device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, i
                 } while (e == 64);
             }
             // nxt[k]: k2 is even, so 2*k2 is the byte offset
-            *reinterpret_cast<int32_t*>(reinterpret_cast<char*>(nxt) + 2 * k2) = val + acc;
+            nxt[k2 >> 1] = (Off)(val + acc);
             // the chunk's first lane in the range stores its ballots: lane 0,
             // unless a bounded range starts inside the chunk
             bool first;
@@ -537,7 +539,10 @@ __device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, i
 // BOUNDED: the pass reads a distance bound per pair (a.bound) and leaves out
 // the cells beyond it; a separate instantiation, so that passes without
 // bounds keep the registers (and occupancy) of the plain level loop.
-template <int G, bool BT_LDS, bool BOUNDED>
+// Off: the type of the LDS offsets, int32_t, or int16_t when twice the
+// largest offset a pass can reach (max_len + dcap) fits (half the LDS, more
+// pairs per CU).
+template <int G, bool BT_LDS, bool BOUNDED, typename Off = int32_t>
 __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, uint64_t* bt_g,
                            int gtid) {
     const int lane = gtid & 63;
@@ -556,8 +561,8 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
     uint64_t* T = slot + sw;
     const uint32_t* P32 = reinterpret_cast<const uint32_t*>(P);
     const uint32_t* T32 = reinterpret_cast<const uint32_t*>(T);
-    int32_t* X = reinterpret_cast<int32_t*>(slot + 2 * sw) + (dcap + 2);
-    int32_t* Y = X + (2 * dcap + 5);
+    Off* X = reinterpret_cast<Off*>(slot + 2 * sw) + (dcap + 2);
+    Off* Y = X + (2 * dcap + 5);
     uint64_t* BT = BT_LDS ? slot + slot_bt_word(sw, dcap) : bt_g;
     // the HBM backtrace slot as a buffer: stores take their offset in a scalar
     const __amdgpu_buffer_rsrc_t BTs =
@@ -595,8 +600,8 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
     const int U = (int)ub;
     bool done = (tk == 0) && (X[0] == 2 * tlen);
     int d = 0;
-    int32_t* cur = X;
-    int32_t* nxt = Y;
+    Off* cur = X;
+    Off* nxt = Y;
     uint32_t base = 0;  // bt_level_base(d): level d's first chunk
     // a search with a bound ends by level U (at most limit); the ranges below
     // are not empty up to there
@@ -608,9 +613,9 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
             // cells within the bound
             const int ilo = max(0, tk + d - (U - d));
             const int ihi = min(2 * d, tk + d + (U - d));
-            block_level<G, BT_LDS, true>(d, ilo, ihi, wave, lane, cur, nxt, P32, T32, plen, tlen, base, BT, BTs);
+            block_level<G, BT_LDS, true, Off>(d, ilo, ihi, wave, lane, cur, nxt, P32, T32, plen, tlen, base, BT, BTs);
         } else {
-            block_level<G, BT_LDS, false>(d, 0, 2 * d, wave, lane, cur, nxt, P32, T32, plen, tlen, base, BT, BTs);
+            block_level<G, BT_LDS, false, Off>(d, 0, 2 * d, wave, lane, cur, nxt, P32, T32, plen, tlen, base, BT, BTs);
         }
         // sentinels for the next level's reads at |k| = d+1, d+2
         if (gtid < 4) {
@@ -620,7 +625,7 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
         group_sync<G>();
         base += (uint32_t)((d >> 5) + 1);
         done = (tka <= d) && (nxt[tk] == 2 * tlen);
-        int32_t* tmp = cur;
+        Off* tmp = cur;
         cur = nxt;
         nxt = tmp;
     }
@@ -1192,7 +1197,7 @@ __global__ __launch_bounds__(64, NC <= 3 ? 8 : 6) void align_reg_kernel(AlignArg
 // ---------------------------------------------------------------------------
 // align kernel: G waves per pair, one pair slot per block
 
-template <int G, bool BT_LDS, bool BOUNDED = false>
+template <int G, bool BT_LDS, bool BOUNDED = false, bool OFF16 = false>
 __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persistent) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t s_base;
@@ -1201,7 +1206,9 @@ __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persiste
     const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
     if (!persistent) {
         const uint32_t item = blockIdx.x;
-        if (item < count) align_pair<G, BT_LDS, BOUNDED>(a, a.list ? a.list[item] : item, lds, btg, tid);
+        if (item < count)
+            align_pair<G, BT_LDS, BOUNDED, typename std::conditional<OFF16, int16_t, int32_t>::type>(
+                a, a.list ? a.list[item] : item, lds, btg, tid);
         return;
     }
     for (;;) {
@@ -1219,7 +1226,8 @@ __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persiste
         if (base >= count) break;
         const uint32_t end = min(base + a.grab, count);
         for (uint32_t it = base; it < end; ++it) {
-            align_pair<G, BT_LDS, BOUNDED>(a, a.list ? a.list[it] : it, lds, btg, tid);
+            align_pair<G, BT_LDS, BOUNDED, typename std::conditional<OFF16, int16_t, int32_t>::type>(
+                a, a.list ? a.list[it] : it, lds, btg, tid);
             group_sync<G>();  // the slot is reused by the next pair
         }
     }
@@ -1849,8 +1857,12 @@ __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, uint32_t val
 template <int G, bool BT_LDS>
 static hipError_t launch_align_t(const AlignArgs& a, bool persistent, uint32_t grid, size_t lds,
                                  hipStream_t stream) {
+    if (a.bound && a.off16) return hipErrorInvalidValue;  // the bounded kernel keeps 32-bit offsets
     if (a.bound && !BT_LDS)
         hipLaunchKernelGGL((align_kernel<G, false, true>), dim3(grid), dim3(64 * G), lds, stream, a,
+                           persistent ? 1 : 0);
+    else if (a.off16 && !BT_LDS)
+        hipLaunchKernelGGL((align_kernel<G, false, false, true>), dim3(grid), dim3(64 * G), lds, stream, a,
                            persistent ? 1 : 0);
     else
         hipLaunchKernelGGL((align_kernel<G, BT_LDS>), dim3(grid), dim3(64 * G), lds, stream, a,
@@ -1911,26 +1923,32 @@ int align_reg_blocks_per_cu(int nc, size_t lds) {
     return blocks;
 }
 
-template <int G, bool BT_LDS, bool BOUNDED = false>
-static const void* align_fn() {
-    return reinterpret_cast<const void*>(&align_kernel<G, BT_LDS, BOUNDED>);
+template <int G>
+static const void* align_fn_variant(int variant) {
+    switch (variant) {
+        case 0: return reinterpret_cast<const void*>(&align_kernel<G, false>);
+        case 1: return reinterpret_cast<const void*>(&align_kernel<G, true>);
+        case 2: return reinterpret_cast<const void*>(&align_kernel<G, false, true>);
+        case 3: return reinterpret_cast<const void*>(&align_kernel<G, false, false, true>);
+        default: return nullptr;
+    }
 }
 
-// variant 0: backtrace in HBM, 1: in LDS, 2: in HBM with distance bounds
+// variant 0: backtrace in HBM, 1: in LDS, 2: in HBM with distance bounds,
+// 3: in HBM with 16-bit offsets
 static const void* align_fn_for(int waves, int variant) {
     switch (waves) {
-        case 1: return variant == 1 ? align_fn<1, true>() : variant ? align_fn<1, false, true>() : align_fn<1, false>();
-        case 2: return variant == 1 ? align_fn<2, true>() : variant ? align_fn<2, false, true>() : align_fn<2, false>();
-        case 4: return variant == 1 ? align_fn<4, true>() : variant ? align_fn<4, false, true>() : align_fn<4, false>();
-        case 8: return variant == 1 ? align_fn<8, true>() : variant ? align_fn<8, false, true>() : align_fn<8, false>();
-        case 16:
-            return variant == 1 ? align_fn<16, true>() : variant ? align_fn<16, false, true>() : align_fn<16, false>();
+        case 1: return align_fn_variant<1>(variant);
+        case 2: return align_fn_variant<2>(variant);
+        case 4: return align_fn_variant<4>(variant);
+        case 8: return align_fn_variant<8>(variant);
+        case 16: return align_fn_variant<16>(variant);
         default: return nullptr;
     }
 }
 
 hipError_t align_set_max_lds(int waves, size_t lds) {
-    for (int b = 0; b < 3; ++b) {
+    for (int b = 0; b < 4; ++b) {
         const void* f = align_fn_for(waves, b);
         if (!f) return hipErrorInvalidValue;
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1939,8 +1957,8 @@ hipError_t align_set_max_lds(int waves, size_t lds) {
     return hipSuccess;
 }
 
-int align_blocks_per_cu(int waves, size_t lds) {
-    const void* f = align_fn_for(waves, 1);
+int align_blocks_per_cu(int waves, size_t lds, int variant) {
+    const void* f = align_fn_for(waves, variant);
     if (!f) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, 64 * waves, lds) != hipSuccess)
