@@ -24,8 +24,9 @@ Policy default_policy() {
     p.buckets.push_back({300u, 1, 63, 1, 16});
     p.buckets.push_back({700u, 1, 127, 1, 4});
     p.buckets.push_back({1200u, 1, 127, 1, 4});
+    p.buckets.push_back({2500u, 2, 768, 0, 1});
     p.buckets.push_back({5000u, 2, 512, 0, 1});
-    p.buckets.push_back({10500u, 4, 1024, 0, 1});
+    p.buckets.push_back({10500u, 2, 1024, 0, 1});
     p.buckets.push_back({0xFFFFFFFFu, 16, 1 << 20, 0, 1});
     return p;
 }
