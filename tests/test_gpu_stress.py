@@ -128,3 +128,36 @@ def test_low_complexity_pairs_all_policies(tmp_path, seed):
         if ref is None:
             ref = res
         assert np.array_equal(ref, res), name
+
+
+def test_sixteen_bit_offsets_near_their_limit(tmp_path):
+    """A pass whose max_len + budget is 16,324 keeps its offsets in 16 bits;
+    texts of 15,300 bases with a 700-900 base insertion drive doubled
+    offsets to within a few hundred of 32,767 (an offset also runs one base a
+    level past the text's end).  Results must equal the oracle's."""
+    rng = np.random.default_rng(77)
+    P, T = [], []
+    for i in range(12):
+        ins = int(rng.integers(700, 900))
+        p = rng.choice(np.frombuffer(BASES, dtype=np.uint8), 15300 - ins)
+        t = p.copy()
+        sub = rng.random(t.size) < 0.01
+        t[sub] = rng.choice(np.frombuffer(BASES, dtype=np.uint8), int(sub.sum()))
+        at = int(rng.integers(0, t.size + 1)) if i % 2 else t.size  # inside, or at the end
+        t = np.concatenate([t[:at], rng.choice(np.frombuffer(BASES, dtype=np.uint8), ins), t[at:]])
+        P.append(p.tobytes())
+        T.append(t.tobytes())
+    print(f"case 16-bit offsets near the limit: {len(P)} pairs, texts 15300 bp", flush=True)
+    d, st = oracle_streams(P, T)
+    assert max(d) <= 1024
+    words, pairs, cb, ml = ewfa.from_sequences(P, T)
+    assert ml + 1024 <= 16383
+    path = str(tmp_path / "off16.policy")
+    with open(path, "w") as f:
+        f.write("4000000000 2 1024 0 1\n")
+    with ewfa.Context(policy_path=path) as ctx:
+        res, cig = ctx.align_host(words, pairs, len(P), cb)
+    streams = ewfa.streams_from(res, pairs, cig)
+    for i in range(len(P)):
+        assert int(res[i]["status"]) == ewfa.PAIR_OK and int(res[i]["distance"]) == int(d[i]), (i, int(res[i]["status"]))
+        assert np.array_equal(streams[i], st[i]), i
