@@ -242,8 +242,8 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
     const int opr1 = ROUNDS > 1 && 64 + lane < dist ? (int)((ow[(64 + lane) >> 5] >> ((lane & 31) << 1)) & 3u) : 0;
     const int nsteps = status == EWFA_PAIR_UNDEFINED ? 0 : dist;
     // step s's diagonal: k_s = tk + (sum of op - 2 over steps s..start-1);
-    // round 1 (steps 64..127) first, its total carried into round 0.  The walk
-    // stops at the first op 0 (a capped path): at most one, in the last round.
+    // steps 64..127 first, their total carried into steps 0..63.  The walk
+    // stops at the first op 0 (a capped path): at most one, in the last 64.
     const int x1 = 64 + lane < start ? opr1 - 2 : 0;
     const int x1_incl = ROUNDS > 1 ? wave_incl_scan_i32(x1) : 0;
     const int tot1 = ROUNDS > 1 ? (int)__builtin_amdgcn_readlane(x1_incl, 63) : 0;
