@@ -273,8 +273,8 @@ int main(int argc, char** argv) {
     ewfa_pairs_file_t f;
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     int rc = ewfa_read_pairs(a.file, (size_t)a.seq_len, num_alignments, (int)std::min(hw, 16u), &f);
-    if (rc == EWFA_ERR_IO) fprintf(stderr, "Could not open the file %s\n", a.file);
-    if (rc == EWFA_ERR_FORMAT) fprintf(stderr, "Invalid sequence format.\n");
+    // the reader's messages are the reference's (utils/sequence_reader.cu:124, :157-158)
+    if (rc == EWFA_ERR_IO || rc == EWFA_ERR_FORMAT) fputs(ewfa_last_error(), stderr);
     if (rc == EWFA_ERR_NOMEM) fatal("Could not allocate memory.\n");
     if (rc != EWFA_OK) {
         fprintf(stderr, "Could not read the sequences from file %s\n", a.file);

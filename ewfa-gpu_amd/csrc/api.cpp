@@ -44,31 +44,15 @@ int fail(int code, const char* fmt, ...) {
         if (r_ != EWFA_OK) return r_; \
     } while (0)
 
-// Bumped whenever a device buffer is replaced: a captured graph that may
-// hold the old address is captured again.
-std::atomic<uint64_t> g_buffer_gen{0};
-
-// Device buffer that only grows.
+// Device buffer of a context that only grows.  Growing it waits for the
+// work queued on its context's streams (the only streams its buffers are used
+// on), so other contexts on the device keep running, and bumps the context's
+// buffer generation: a captured graph that may hold the old address is
+// captured again.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
-    int reserve(size_t n) {
-        if (n <= bytes) return EWFA_OK;
-        // queued work may still use the old buffer
-        if (p) (void)hipDeviceSynchronize();
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        if (n == 0) return EWFA_OK;
-        hipError_t e = hipMalloc(&p, n);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(EWFA_ERR_NOMEM, "hipMalloc(%zu): %s", n, hipGetErrorString(e));
-        }
-        bytes = n;
-        g_buffer_gen.fetch_add(1, std::memory_order_relaxed);
-        return EWFA_OK;
-    }
+    int reserve(ewfa_ctx* c, size_t n);
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -121,7 +105,8 @@ struct ewfa_ctx {
     } graph_key{};
     bool use_graph = true;
     int policy_gen = 0, graph_policy_gen = -1;
-    uint64_t graph_buffer_gen = 0;  // g_buffer_gen when the graph was captured
+    uint64_t buffer_gen = 0;        // bumped whenever one of the buffers is replaced
+    uint64_t graph_buffer_gen = 0;  // buffer_gen when the graph was captured
     // op scratch supplied by the caller for one call (the job server's arena)
     uint64_t* ops_ext = nullptr;
     size_t ops_ext_bytes = 0;
@@ -147,6 +132,27 @@ struct ewfa_ctx {
 };
 
 namespace {
+
+int DevBuf::reserve(ewfa_ctx* c, size_t n) {
+    if (n <= bytes) return EWFA_OK;
+    if (p) {
+        // work queued on the context's streams may still use the old buffer
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        if (c->up) (void)hipStreamSynchronize(c->up);
+        (void)hipFree(p);
+    }
+    p = nullptr;
+    bytes = 0;
+    if (n == 0) return EWFA_OK;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return fail(EWFA_ERR_NOMEM, "hipMalloc(%zu): %s", n, hipGetErrorString(e));
+    }
+    bytes = n;
+    ++c->buffer_gen;
+    return EWFA_OK;
+}
 
 int set_device(ewfa_ctx* c) {
     HIPCHK(hipSetDevice(c->device));
@@ -204,10 +210,8 @@ int finish_pass(ewfa_ctx* c, Pass& p, uint32_t items) {
                   : ewfa::lds_bytes(p.seq_words, p.dcap, p.bt_lds, p.off16);
     if (p.lds > lds_budget(c))
         return fail(EWFA_ERR_RANGE, "pass needs %zu bytes of LDS (device has %zu)", p.lds, c->lds_max);
-    if (p.lds > 65536) {
-        if (p.reg) return fail(EWFA_ERR_RANGE, "register pass above 64 KiB of LDS");
-        HIPCHK(ewfa::align_set_max_lds(p.waves, p.lds));
-    }
+    // (the block kernels' LDS limit was raised to the budget at ewfa_create)
+    if (p.lds > 65536 && p.reg) return fail(EWFA_ERR_RANGE, "register pass above 64 KiB of LDS");
     if (p.persistent) {
         int bpc = p.reg ? ewfa::align_reg_blocks_per_cu(p.reg, p.lds)
                         : ewfa::align_blocks_per_cu(p.waves, p.lds, p.off16 ? 3 : 1);
@@ -356,11 +360,11 @@ int reserve_scratch(ewfa_ctx* c, size_t n, uint32_t max_len) {
         if (c->ops_ext_bytes < ops_bytes)
             return fail(EWFA_ERR_RANGE, "op scratch of %zu bytes, %zu needed", c->ops_ext_bytes, ops_bytes);
     } else {
-        CHK(c->ops.reserve(ops_bytes));
+        CHK(c->ops.reserve(c, ops_bytes));
     }
-    CHK(c->list_a.reserve((size_t)std::max<size_t>(n, 1) * 4));
-    CHK(c->list_b.reserve((size_t)std::max<size_t>(n, 1) * 4));
-    CHK(c->list_fb.reserve((size_t)std::max<size_t>(n, 1) * 4));
+    CHK(c->list_a.reserve(c, (size_t)std::max<size_t>(n, 1) * 4));
+    CHK(c->list_b.reserve(c, (size_t)std::max<size_t>(n, 1) * 4));
+    CHK(c->list_fb.reserve(c, (size_t)std::max<size_t>(n, 1) * 4));
     return EWFA_OK;
 }
 
@@ -372,17 +376,17 @@ uint32_t runs_slot_of(const Pass& p) {
 }
 
 int reserve_bt(ewfa_ctx* c, const Pass& p) {
-    if (p.reg) return c->runs.reserve((size_t)runs_slot_of(p) * p.grid);
+    if (p.reg) return c->runs.reserve(c, (size_t)runs_slot_of(p) * p.grid);
     if (p.bt_lds) return EWFA_OK;
     const size_t slot = 2u * (size_t)ewfa::bt_chunks((uint32_t)p.dcap);
-    return c->bt.reserve(slot * p.grid * 8);
+    return c->bt.reserve(c, slot * p.grid * 8);
 }
 
 int reserve_plan(ewfa_ctx* c, const Plan& pl, size_t n) {
     for (int b = 0; b < pl.nb; ++b) CHK(reserve_bt(c, pl.pass[b]));
     for (int k = 0; k < pl.chain.np; ++k) CHK(reserve_bt(c, pl.chain.pass[k]));
-    if (pl.nb > 1) CHK(c->buckets.reserve((size_t)pl.nb * std::max<size_t>(n, 1) * 4));
-    CHK(c->bound.reserve(std::max<size_t>(n, 1) * 4));
+    if (pl.nb > 1) CHK(c->buckets.reserve(c, (size_t)pl.nb * std::max<size_t>(n, 1) * 4));
+    CHK(c->bound.reserve(c, std::max<size_t>(n, 1) * 4));
     return EWFA_OK;
 }
 
@@ -781,7 +785,7 @@ int pipeline_enqueue(ewfa_ctx* c, const Chunk& ch, size_t k) {
     CHK(reserve_plan(c, pl, n));
     CHK(enqueue_plan(c, pl, d_words, d_pairs + ch.lo, (uint32_t)n, ch.max_len, d_results + ch.lo, d_cigars,
                      c->stream, false));
-    CHK(c->scan.reserve(16 * ((n + ewfa::COMPACT_BLOCK - 1) / ewfa::COMPACT_BLOCK)));
+    CHK(c->scan.reserve(c, 16 * ((n + ewfa::COMPACT_BLOCK - 1) / ewfa::COMPACT_BLOCK)));
     HIPCHK(ewfa::launch_compact(d_pairs + ch.lo, d_results + ch.lo, d_cigars, n, c->scan.as<uint64_t>(), st.dense_dev,
                                 c->stream));
     HIPCHK(hipMemcpyAsync(st.res, d_results + ch.lo, n * sizeof(ewfa_result_t), hipMemcpyDeviceToHost, c->stream));
@@ -805,12 +809,12 @@ int pipeline_complete(ewfa_ctx* c, const Chunk& ch, size_t k, const ewfa_pair_t*
 int run_pipeline(ewfa_ctx* c, const uint64_t* h_words, size_t n_words, const ewfa_pair_t* h_pairs, size_t n,
                  ewfa_result_t* h_results, int8_t* h_cigars, size_t cigar_bytes) {
     if (h_words) {
-        CHK(c->h_words.reserve(std::max<size_t>(n_words, 1) * 8));
+        CHK(c->h_words.reserve(c, std::max<size_t>(n_words, 1) * 8));
         c->resident_words = 0;  // only the words these pairs read go up
     }
-    CHK(c->h_pairs.reserve(n * sizeof(ewfa_pair_t)));
-    CHK(c->h_results.reserve(n * sizeof(ewfa_result_t)));
-    CHK(c->h_cigars.reserve(std::max<size_t>(cigar_bytes, 4)));
+    CHK(c->h_pairs.reserve(c, n * sizeof(ewfa_pair_t)));
+    CHK(c->h_results.reserve(c, n * sizeof(ewfa_result_t)));
+    CHK(c->h_cigars.reserve(c, std::max<size_t>(cigar_bytes, 4)));
     // earlier work on the context's stream finishes before the uploads land
     HIPCHK(hipEventRecord(c->ev_in, c->stream));
     HIPCHK(hipStreamWaitEvent(c->up, c->ev_in, 0));
@@ -875,6 +879,16 @@ int run_pipeline(ewfa_ctx* c, const uint64_t* h_words, size_t n_words, const ewf
 }
 
 namespace ewfa {
+int set_error(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
 size_t op_scratch_bytes(size_t n, uint32_t max_len) { return n * (size_t)ops_words(max_len + 1) * 8; }
 
 int align_device_with_scratch(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t* d_pairs, size_t n,
@@ -919,8 +933,8 @@ int enqueue_compact(ewfa_ctx_t* c, const ewfa_pair_t* d_pairs, const ewfa_result
     if (n == 0) return EWFA_OK;
     CHK(set_device(c));
     const size_t nb = (n + COMPACT_BLOCK - 1) / COMPACT_BLOCK;
-    CHK(c->dense.reserve(std::max<size_t>(bound, 4)));
-    CHK(c->scan.reserve(nb * 16));
+    CHK(c->dense.reserve(c, std::max<size_t>(bound, 4)));
+    CHK(c->scan.reserve(c, nb * 16));
     HIPCHK(launch_compact(d_pairs, d_results, d_cigars, n, c->scan.as<uint64_t>(), c->dense.as<int8_t>(), s));
     return EWFA_OK;
 }
@@ -1018,7 +1032,13 @@ int ewfa_create(ewfa_ctx_t** out, const ewfa_opts_t* opts) {
             if (c->lds_max == 0) c->lds_max = 65536;
         }
     }
-    if (rc == EWFA_OK) rc = ewfa::load_policy(c->opts.policy_path, c->policy);
+    if (rc == EWFA_OK && lds_budget(c) > 65536 && ewfa::align_set_max_lds(lds_budget(c)) != hipSuccess)
+        rc = fail(EWFA_ERR_HIP, "could not raise the block kernels' LDS limit to %zu bytes", lds_budget(c));
+    if (rc == EWFA_OK) {
+        rc = ewfa::load_policy(c->opts.policy_path, c->policy);
+        if (rc == EWFA_ERR_IO) fail(rc, "could not open the policy file %s", c->opts.policy_path);
+        if (rc == EWFA_ERR_FORMAT) fail(rc, "bad policy file %s", c->opts.policy_path);
+    }
     if (rc == EWFA_OK && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         rc = fail(EWFA_ERR_HIP, "hipStreamCreate failed");
     if (rc == EWFA_OK) {
@@ -1026,7 +1046,7 @@ int ewfa_create(ewfa_ctx_t** out, const ewfa_opts_t* opts) {
             hipEventCreate(&c->ev_a1) != hipSuccess || hipEventCreate(&c->ev_e1) != hipSuccess)
             rc = fail(EWFA_ERR_HIP, "hipEventCreate failed");
     }
-    if (rc == EWFA_OK) rc = c->ctrl.reserve(CTRL_WORDS * 4);
+    if (rc == EWFA_OK) rc = c->ctrl.reserve(c, CTRL_WORDS * 4);
     if (rc == EWFA_OK && hipHostMalloc((void**)&c->host_ctrl, CTRL_WORDS * 4, hipHostMallocDefault) != hipSuccess)
         rc = fail(EWFA_ERR_NOMEM, "hipHostMalloc failed");
     if (rc != EWFA_OK) {
@@ -1187,7 +1207,7 @@ int ewfa_align_device(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t*
         if (c->use_graph) {
             // with timing: two graphs (align, emit) with events between them
             const int nparts = timing ? 2 : 1;
-            const uint64_t gen = g_buffer_gen.load(std::memory_order_relaxed);
+            const uint64_t gen = c->buffer_gen;
             if (!(c->graph && c->graph_key == key && c->graph_policy_gen == c->policy_gen &&
                   c->graph_buffer_gen == gen)) {
                 for (hipGraphExec_t* ge : {&c->graph, &c->graph2})
@@ -1269,7 +1289,7 @@ int ewfa_upload_words(ewfa_ctx_t* c, const uint64_t* h_words, size_t n_words) {
     if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
     if (!h_words && n_words) return fail(EWFA_ERR_ARG, "null buffer");
     CHK(set_device(c));
-    CHK(c->h_words.reserve(std::max<size_t>(n_words, 1) * 8));
+    CHK(c->h_words.reserve(c, std::max<size_t>(n_words, 1) * 8));
     if (n_words) {
         HIPCHK(hipMemcpyAsync(c->h_words.p, h_words, n_words * 8, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1302,9 +1322,9 @@ int ewfa_align_resident(ewfa_ctx_t* c, const ewfa_pair_t* h_pairs, size_t n, ewf
             return fail(EWFA_ERR_ARG, "pair %zu has a bad CIGAR region", i);
     }
     CHK(set_device(c));
-    CHK(c->h_pairs.reserve(n * sizeof(ewfa_pair_t)));
-    CHK(c->h_results.reserve(n * sizeof(ewfa_result_t)));
-    CHK(c->h_cigars.reserve(std::max<size_t>(cigar_bytes, 4)));
+    CHK(c->h_pairs.reserve(c, n * sizeof(ewfa_pair_t)));
+    CHK(c->h_results.reserve(c, n * sizeof(ewfa_result_t)));
+    CHK(c->h_cigars.reserve(c, std::max<size_t>(cigar_bytes, 4)));
     hipStream_t s = c->stream;
     HIPCHK(hipMemcpyAsync(c->h_pairs.p, h_pairs, n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, s));
     CHK(ewfa_align_device(c, c->h_words.as<uint64_t>(), c->h_pairs.as<ewfa_pair_t>(), n, max_len,

@@ -206,6 +206,8 @@ hipError_t launch_compact(const ewfa_pair_t* pairs, const ewfa_result_t* results
 // Resident blocks per CU of a block-kernel variant (0: backtrace in HBM, 1: in
 // LDS, 2: with distance bounds, 3: with 16-bit offsets).
 int align_blocks_per_cu(int waves, size_t lds, int variant = 1);
-hipError_t align_set_max_lds(int waves, size_t lds);
+// Raises the dynamic LDS limit of every block-kernel variant to `lds` bytes
+// (never lowers it).
+hipError_t align_set_max_lds(size_t lds);
 
 }  // namespace ewfa

@@ -1,9 +1,14 @@
 // host.cpp -- CPU-side parts of libewfa: packing, CIGAR text, the validity
 // check, the pair-file reader and the synthetic workload generator.
+#include <fcntl.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <time.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <string>
@@ -30,6 +35,12 @@ void parallel_for(size_t n, int threads, F&& f) {
         pool.emplace_back([&f, lo, hi] { f(lo, hi); });
     }
     for (auto& th : pool) th.join();
+}
+
+double now_s() {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
 }
 
 }  // namespace
@@ -236,100 +247,140 @@ int ewfa_read_pairs(const char* path, size_t seq_len, size_t n, int threads, ewf
     if (!out) return EWFA_ERR_ARG;
     memset(out, 0, sizeof *out);
     out->n = n;
+    threads = std::max(1, threads);
     const size_t nn = std::max<size_t>(n, 1);
     out->pairs = (ewfa_pair_t*)calloc(nn, sizeof(ewfa_pair_t));
     out->ascii_off = (uint64_t*)calloc(2 * nn, sizeof(uint64_t));
     if (!out->pairs || !out->ascii_off) {
         ewfa_free_pairs(out);
-        return EWFA_ERR_NOMEM;
+        return ewfa::set_error(EWFA_ERR_NOMEM, "Could not allocate memory.\n");
     }
-    std::string data;
-    if (seq_len != 0) {  // -l 0 reads nothing (:109-112)
-        FILE* f = fopen(path, "rb");
-        if (!f) {
+    const double t0 = now_s();
+    const char* data = nullptr;
+    size_t size = 0;
+    if (seq_len != 0) {  // -l 0 reads nothing (utils/sequence_reader.cu:109-112)
+        const int fd = open(path, O_RDONLY);
+        struct stat st;
+        if (fd < 0 || fstat(fd, &st) != 0 || !S_ISREG(st.st_mode)) {
+            if (fd >= 0) close(fd);
             ewfa_free_pairs(out);
-            return EWFA_ERR_IO;
+            return ewfa::set_error(EWFA_ERR_IO, "Could not open the file %s\n", path);
         }
-        char buf[1 << 16];
-        size_t got;
-        while ((got = fread(buf, 1, sizeof buf, f)) > 0) data.append(buf, got);
-        fclose(f);
+        size = (size_t)st.st_size;
+        if (size) {
+            void* m = mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+            if (m == MAP_FAILED) {
+                close(fd);
+                ewfa_free_pairs(out);
+                return ewfa::set_error(EWFA_ERR_IO, "Could not open the file %s\n", path);
+            }
+            out->map = m;
+            out->map_len = size;
+            data = static_cast<const char*>(m);
+        }
+        close(fd);
     }
-    // line boundaries of the first 2n lines (a line includes its '\n'; the
-    // last one may lack it)
-    std::vector<size_t> starts, lens;
-    size_t pos = 0;
-    while (starts.size() < 2 * n && pos < data.size()) {
-        const void* nl = memchr(data.data() + pos, '\n', data.size() - pos);
-        const size_t end = nl ? (size_t)((const char*)nl - data.data()) + 1 : data.size();
-        starts.push_back(pos);
-        lens.push_back(end - pos);
-        pos = end;
+    // The first 2n lines, [starts[i], ends[i]) each (a line includes its
+    // '\n'; the last one of the file may lack it).  The file is scanned in
+    // rounds of `threads` blocks, each block's newlines found by one thread,
+    // until 2n lines are known or the file ends.
+    std::vector<size_t> starts, ends;
+    constexpr size_t BLOCK = (size_t)8 << 20;
+    size_t scan = 0;  // first byte not yet scanned
+    size_t line = 0;  // start of the next line
+    while (starts.size() < 2 * n && line < size) {
+        const size_t nb = std::min<size_t>((size_t)threads, (size - scan + BLOCK - 1) / BLOCK);
+        std::vector<std::vector<size_t>> nl(nb);
+        parallel_for(nb, (int)nb, [&](size_t lo, size_t hi) {
+            for (size_t k = lo; k < hi; ++k) {
+                const size_t b0 = scan + k * BLOCK, b1 = std::min(size, b0 + BLOCK);
+                for (const char* q = data + b0; q < data + b1;) {
+                    const void* f = memchr(q, '\n', (size_t)(data + b1 - q));
+                    if (!f) break;
+                    nl[k].push_back((size_t)((const char*)f - data));
+                    q = (const char*)f + 1;
+                }
+            }
+        });
+        for (size_t k = 0; k < nb && starts.size() < 2 * n; ++k)
+            for (size_t i = 0; i < nl[k].size() && starts.size() < 2 * n; ++i) {
+                starts.push_back(line);
+                ends.push_back(nl[k][i] + 1);
+                line = nl[k][i] + 1;
+            }
+        scan = std::min(size, scan + nb * BLOCK);
+        if (scan == size && starts.size() < 2 * n && line < size) {
+            starts.push_back(line);  // the last line has no newline
+            ends.push_back(size);
+            line = size;
+        }
     }
-    int rc = EWFA_OK;
+    // markers: '>' on pattern lines, '<' on text lines; the reference stops at
+    // the first bad line (utils/sequence_reader.cu:154-176)
     size_t lines = starts.size();
+    int rc = EWFA_OK;
     for (size_t i = 0; i < lines; ++i) {
         const char marker = (i % 2 == 0) ? '>' : '<';
         if (data[starts[i]] != marker) {
-            rc = EWFA_ERR_FORMAT;
-            lines = i;  // the reference stops at the first bad line
+            const std::string text(data + starts[i], ends[i] - starts[i]);
+            rc = ewfa::set_error(EWFA_ERR_FORMAT, "Invalid sequence format on line %d.\n    %s\n", (int)i,
+                                 text.c_str());
+            lines = i;
             break;
         }
     }
-    // sequences: bytes [1, len-1) of each line, i.e. the newline is assumed
-    std::vector<size_t> slen(2 * nn, 0);
-    size_t ascii_total = 0;
-    for (size_t i = 0; i < lines; ++i) {
-        slen[i] = lens[i] >= 2 ? lens[i] - 2 : 0;
-        ascii_total += slen[i];
-    }
-    out->ascii = (char*)malloc(ascii_total + 1);
-    if (!out->ascii) {
+    out->scan_s = now_s() - t0;
+    // sequences: bytes [1, len-1) of each line, i.e. the newline is assumed;
+    // they stay in the mapping, where ascii_off points
+    if (out->map) {
+        out->ascii = static_cast<char*>(out->map);
+    } else if (!(out->ascii = (char*)calloc(1, 1))) {
         ewfa_free_pairs(out);
-        return EWFA_ERR_NOMEM;
+        return ewfa::set_error(EWFA_ERR_NOMEM, "Could not allocate memory.\n");
     }
-    size_t words_total = 0;
+    std::vector<uint32_t> slen(2 * nn, 0);
     std::vector<uint64_t> woff(2 * nn, 0);
-    size_t aoff = 0;
+    size_t words_total = 0;
     uint32_t max_len = 0;
     for (size_t j = 0; j < 2 * n; ++j) {
-        out->ascii_off[j] = aoff;
+        if (j < lines) {
+            const size_t len = ends[j] - starts[j];
+            slen[j] = len >= 2 ? (uint32_t)(len - 2) : 0u;
+            out->ascii_off[j] = starts[j] + 1;
+        }
         woff[j] = words_total;
-        aoff += slen[j];
         words_total += ewfa_packed_words(slen[j]);
-        max_len = std::max<uint32_t>(max_len, (uint32_t)slen[j]);
+        max_len = std::max(max_len, slen[j]);
     }
     out->words = (uint64_t*)malloc(std::max<size_t>(words_total, 1) * 8);
     if (!out->words) {
         ewfa_free_pairs(out);
-        return EWFA_ERR_NOMEM;
+        return ewfa::set_error(EWFA_ERR_NOMEM, "Could not allocate memory.\n");
     }
     out->n_words = words_total;
     out->max_len = max_len;
-    const char* src = data.data();
+    const double t1 = now_s();
     parallel_for(2 * n, threads, [&](size_t lo, size_t hi) {
-        for (size_t j = lo; j < hi; ++j) {
-            const char* s = (j < lines) ? src + starts[j] + 1 : "";
-            memcpy(out->ascii + out->ascii_off[j], s, slen[j]);
-            ewfa_pack_host(s, slen[j], out->words + woff[j]);
-        }
+        for (size_t j = lo; j < hi; ++j)
+            ewfa_pack_host(out->ascii + out->ascii_off[j], slen[j], out->words + woff[j]);
     });
+    out->pack_s = now_s() - t1;
     for (size_t i = 0; i < n; ++i) {
         ewfa_pair_t& p = out->pairs[i];
         p.pattern_off = woff[2 * i];
         p.text_off = woff[2 * i + 1];
-        p.plen = (uint32_t)slen[2 * i];
-        p.tlen = (uint32_t)slen[2 * i + 1];
+        p.plen = slen[2 * i];
+        p.tlen = slen[2 * i + 1];
     }
-    if (rc != EWFA_OK) return rc;
-    return EWFA_OK;
+    return rc;
 }
 
 void ewfa_free_pairs(ewfa_pairs_file_t* f) {
     if (!f) return;
     free(f->words);
     free(f->pairs);
-    free(f->ascii);
+    if (f->map) munmap(f->map, f->map_len);
+    else free(f->ascii);
     free(f->ascii_off);
     memset(f, 0, sizeof *f);
 }

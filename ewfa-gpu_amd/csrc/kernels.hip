@@ -23,6 +23,7 @@
 //                            (src/kernels.cu:313-394).
 //
 // Integer work only; nothing here touches MFMA.
+#include <mutex>
 #include <type_traits>
 
 #include "ewfa_internal.h"
@@ -1947,13 +1948,22 @@ static const void* align_fn_for(int waves, int variant) {
     }
 }
 
-hipError_t align_set_max_lds(int waves, size_t lds) {
-    for (int b = 0; b < 4; ++b) {
-        const void* f = align_fn_for(waves, b);
-        if (!f) return hipErrorInvalidValue;
-        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
+// The attribute is per kernel and process-wide, so it is only ever raised
+// (under a lock): no thread can lower it below a pass another thread is
+// about to launch.
+hipError_t align_set_max_lds(size_t lds) {
+    static std::mutex m;
+    static size_t done = 0;
+    std::lock_guard<std::mutex> lk(m);
+    if (lds <= done) return hipSuccess;
+    for (int waves : {1, 2, 4, 8, 16})
+        for (int b = 0; b < 4; ++b) {
+            const void* f = align_fn_for(waves, b);
+            if (!f) return hipErrorInvalidValue;
+            hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+    done = lds;
     return hipSuccess;
 }
 

@@ -50,7 +50,9 @@ int load_policy(const char* path, Policy& out) {
                              bt ? 1 : 0, grab});
     }
     fclose(f);
-    if (p.buckets.empty()) return EWFA_ERR_FORMAT;
+    // a plan holds at most MAX_BUCKETS passes; a longer file would leave its
+    // last buckets (and the pairs bucketed into them) without a pass
+    if (p.buckets.empty() || p.buckets.size() > (size_t)MAX_BUCKETS) return EWFA_ERR_FORMAT;
     p.buckets.back().len_hi = 0xFFFFFFFFu;  // the last bucket takes every longer pair
     out = p;
     return EWFA_OK;
@@ -67,3 +69,15 @@ int save_policy(const char* path, const Policy& p) {
 }
 
 }  // namespace ewfa
+
+extern "C" int ewfa_policy_read(const char* path, ewfa_bucket_policy_t* out, int cap) {
+    ewfa::Policy p;
+    const int rc = ewfa::load_policy(path, p);
+    if (rc != EWFA_OK) return rc;
+    const int n = (int)p.buckets.size();
+    for (int i = 0; i < n && i < cap && out; ++i) {
+        const ewfa::BucketPolicy& b = p.buckets[(size_t)i];
+        out[i] = ewfa_bucket_policy_t{b.len_hi, b.waves, b.dcap, b.bt_lds, b.grab};
+    }
+    return n;
+}

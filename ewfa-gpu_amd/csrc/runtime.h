@@ -10,6 +10,10 @@
 
 namespace ewfa {
 
+// Sets the calling thread's ewfa_last_error() message (printf format) and
+// returns `code`.
+int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
 // Op scratch bytes an align call over n pairs up to max_len bases needs.
 size_t op_scratch_bytes(size_t n, uint32_t max_len);
 

@@ -50,6 +50,11 @@ class CliArgs(ctypes.Structure):
                 ("plain", ctypes.c_int)]
 
 
+class BucketPolicy(ctypes.Structure):
+    _fields_ = [("len_hi", ctypes.c_uint32), ("waves", ctypes.c_int32), ("dcap", ctypes.c_int32),
+                ("bt_lds", ctypes.c_int32), ("grab", ctypes.c_int32)]
+
+
 class ServerOpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("arena_bytes", ctypes.c_size_t),
                 ("max_distance", ctypes.c_int), ("policy_path", ctypes.c_char_p),
@@ -66,7 +71,9 @@ class ServerStats(ctypes.Structure):
 class PairsFile(ctypes.Structure):
     _fields_ = [("n", ctypes.c_size_t), ("n_words", ctypes.c_size_t), ("words", ctypes.POINTER(ctypes.c_uint64)),
                 ("pairs", ctypes.c_void_p), ("ascii", ctypes.POINTER(ctypes.c_char)),
-                ("ascii_off", ctypes.POINTER(ctypes.c_uint64)), ("max_len", ctypes.c_uint32)]
+                ("ascii_off", ctypes.POINTER(ctypes.c_uint64)), ("max_len", ctypes.c_uint32),
+                ("map", ctypes.c_void_p), ("map_len", ctypes.c_size_t), ("scan_s", ctypes.c_double),
+                ("pack_s", ctypes.c_double)]
 
 
 # every symbol include/ewfa.h declares
@@ -80,7 +87,7 @@ EXPORTS = [
     "ewfa_device_alloc", "ewfa_device_free", "ewfa_memcpy", "ewfa_device_synchronize",
     "ewfa_server_create", "ewfa_server_destroy", "ewfa_server_submit", "ewfa_server_poll", "ewfa_server_wait",
     "ewfa_server_job_latency", "ewfa_server_stats", "ewfa_autotune", "ewfa_host_alloc", "ewfa_host_free",
-    "ewfa_context_memory",
+    "ewfa_context_memory", "ewfa_policy_read",
 ]
 
 _lib = None
@@ -140,6 +147,7 @@ def lib():
     L.ewfa_server_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
     L.ewfa_autotune.argtypes = [ctypes.c_int, _vp, _sz, _vp, _sz, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
                                 ctypes.c_char_p, _sz]
+    L.ewfa_policy_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(BucketPolicy), ctypes.c_int]
     _lib = L
     return L
 
@@ -152,6 +160,16 @@ def _check(rc):
     if rc != OK:
         raise EwfaError(f"libewfa error {rc}: {lib().ewfa_last_error().decode(errors='replace')}")
     return rc
+
+
+def read_policy(path):
+    """The buckets of a policy file as ewfa_create reads it (None: the
+    built-in table), as (len_hi, waves, dcap, bt_lds, grab) tuples."""
+    buf = (BucketPolicy * 16)()
+    n = lib().ewfa_policy_read(path.encode() if path else None, buf, 16)
+    if n < 0:
+        raise EwfaError(f"libewfa error {n}: policy file {path!r} rejected")
+    return [(b.len_hi, b.waves, b.dcap, b.bt_lds, b.grab) for b in buf[:n]]
 
 
 def device_count():

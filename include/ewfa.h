@@ -261,6 +261,21 @@ int ewfa_autotune(int device, const uint64_t *h_words, size_t n_words, const ewf
                   size_t n, const uint32_t *bucket_hi, int nb, int reps, const char *out_path,
                   char *report, size_t report_cap);
 
+/* One bucket of a launch policy: pairs with max(plen, tlen) <= len_hi run
+ * with `waves` waves per pair, a wavefront budget of dcap levels (pairs
+ * beyond it go to deferral passes), the backtrace in LDS (bt_lds 1) or HBM,
+ * and `grab` pairs per queue pull. */
+typedef struct ewfa_bucket_policy {
+    uint32_t len_hi;
+    int32_t waves, dcap, bt_lds, grab;
+} ewfa_bucket_policy_t;
+
+/* Reads a policy file as ewfa_create would (NULL or "": the built-in table)
+ * into out[0..cap) and returns its bucket count, or EWFA_ERR_IO /
+ * EWFA_ERR_FORMAT (no valid line, or more than the 8 buckets a plan holds).
+ * CPU only; an addition (the reference has no tuner). */
+int ewfa_policy_read(const char *path, ewfa_bucket_policy_t *out, int cap);
+
 /* ---- host helpers (CPU, no device needed) ------------------------------- */
 
 /* ASCII -> packed words on the host (same layout as ewfa_pack_ascii). */
@@ -304,20 +319,30 @@ int ewfa_check_cigar(const char *pattern, size_t plen, const char *text, size_t 
 /* Pairs read from a '>'/'<' file (SequenceReader, utils/sequence_reader.cu:108-201)
  * with its quirks: a line's bytes [1, len-1) are kept (a last line without a
  * newline loses a byte, '\r' stays), missing pairs have length 0, reading
- * stops after n pairs.  Sequences are stored packed, ready for upload. */
+ * stops after n pairs.  Sequences are stored packed, ready for upload.  One
+ * deliberate difference: the reference copies each line into a fixed slot of
+ * 2*l+4-(2*l mod 4) bytes (utils/sequence_reader.h:66-67) and a longer line
+ * runs into the next slot; here every sequence keeps its own bytes. */
 typedef struct ewfa_pairs_file {
     size_t n;              /* pairs (as requested by -n) */
     size_t n_words;        /* packed words used */
     uint64_t *words;       /* packed sequences (host) */
     ewfa_pair_t *pairs;    /* descriptors; cigar_off/cigar_cap left 0 */
-    char *ascii;           /* raw bytes, pattern i then text i, back to back */
+    char *ascii;           /* raw bytes: sequence j at ascii + ascii_off[j]
+                              (the file itself, mapped read-only) */
     uint64_t *ascii_off;   /* 2n byte offsets into ascii */
     uint32_t max_len;      /* longest sequence read */
+    void *map;             /* the file mapping (owned; ewfa_free_pairs) */
+    size_t map_len;
+    double scan_s, pack_s; /* seconds spent finding lines and packing */
 } ewfa_pairs_file_t;
 
 /* seq_len is the reference's -l (it only gates reading: 0 reads nothing,
- * src/utils/sequence_reader.cu:109-112).  threads packs in parallel.
- * Returns EWFA_OK, EWFA_ERR_IO, EWFA_ERR_FORMAT or EWFA_ERR_NOMEM. */
+ * utils/sequence_reader.cu:109-112).  The file is mapped, its line starts are
+ * found and its sequences packed on `threads` host threads.  Returns EWFA_OK,
+ * EWFA_ERR_IO, EWFA_ERR_FORMAT (ewfa_last_error() then holds the reference's
+ * message, "Invalid sequence format on line <i>." and the line, from
+ * utils/sequence_reader.cu:157-158) or EWFA_ERR_NOMEM. */
 int ewfa_read_pairs(const char *path, size_t seq_len, size_t n, int threads,
                     ewfa_pairs_file_t *out);
 void ewfa_free_pairs(ewfa_pairs_file_t *f);

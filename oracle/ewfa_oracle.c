@@ -380,6 +380,7 @@ typedef struct {
     int32_t *dist, *slen;
     int8_t *streams;
     int64_t stride;
+    int text;
     int64_t ok;
 } or_batch_job;
 
@@ -395,7 +396,7 @@ static void *or_batch_worker(void *arg) {
         if (4 * need > tcap) { tcap = 4 * need; text = (char *)realloc(text, (size_t)tcap); }
         int d = 0, sl = 0;
         int r = or_pair_full(j->pat + j->pat_off[i], pl, j->txt + j->txt_off[i], tl,
-                             j->max_distance, &d, stream, scap, &sl, text, tcap);
+                             j->max_distance, &d, stream, scap, &sl, j->text ? text : NULL, tcap);
         if (r == 1) j->ok++;
         if (j->dist) j->dist[i] = d;
         if (j->slen) j->slen[i] = sl;
@@ -412,6 +413,14 @@ int64_t or_batch_streams(const char *pat, const int64_t *pat_off, const int32_t 
                          const char *txt, const int64_t *txt_off, const int32_t *tlen,
                          int64_t n, int max_distance, int threads,
                          int32_t *dist, int32_t *slen, int8_t *streams, int64_t stride) {
+    return or_batch_text(pat, pat_off, plen, txt, txt_off, tlen, n, max_distance, threads, 1,
+                         dist, slen, streams, stride);
+}
+
+int64_t or_batch_text(const char *pat, const int64_t *pat_off, const int32_t *plen,
+                      const char *txt, const int64_t *txt_off, const int32_t *tlen,
+                      int64_t n, int max_distance, int threads, int text,
+                      int32_t *dist, int32_t *slen, int8_t *streams, int64_t stride) {
     if (threads < 1) threads = 1;
     or_batch_job *jobs = (or_batch_job *)calloc((size_t)threads, sizeof(or_batch_job));
     pthread_t *th = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
@@ -420,7 +429,7 @@ int64_t or_batch_streams(const char *pat, const int64_t *pat_off, const int32_t 
         or_batch_job *j = &jobs[i];
         j->pat = pat; j->txt = txt; j->pat_off = pat_off; j->txt_off = txt_off;
         j->plen = plen; j->tlen = tlen; j->max_distance = max_distance;
-        j->dist = dist; j->slen = slen; j->streams = streams; j->stride = stride;
+        j->dist = dist; j->slen = slen; j->streams = streams; j->stride = stride; j->text = text;
         j->lo = per * i;
         j->hi = (i == threads - 1) ? n : per * (i + 1); /* last takes the rest */
         if (i > 0) pthread_create(&th[i], NULL, or_batch_worker, j);

@@ -119,6 +119,14 @@ int64_t or_batch_streams(const char *pat, const int64_t *pat_off, const int32_t 
                          int64_t n, int max_distance, int threads,
                          int32_t *dist, int32_t *slen, int8_t *streams, int64_t stride);
 
+/* or_batch_streams with the CIGAR text step on (text = 1, the reference's
+ * -p path: Cigars::generate_ascii_cigar, src/wavefront.cuh:146-184, called at
+ * src/wavefront.cu:293-300) or off (text = 0: align, stream and check only). */
+int64_t or_batch_text(const char *pat, const int64_t *pat_off, const int32_t *plen,
+                      const char *txt, const int64_t *txt_off, const int32_t *tlen,
+                      int64_t n, int max_distance, int threads, int text,
+                      int32_t *dist, int32_t *slen, int8_t *streams, int64_t stride);
+
 #ifdef __cplusplus
 }
 #endif

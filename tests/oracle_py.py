@@ -58,6 +58,10 @@ def lib():
                                ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                ctypes.c_void_p, ctypes.c_void_p]
         L.or_batch.restype = ctypes.c_int64
+        L.or_batch_text.argtypes = [ctypes.c_void_p] * 6 + [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                             ctypes.c_int64]
+        L.or_batch_text.restype = ctypes.c_int64
         L.or_batch_streams.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_int64, ctypes.c_int, ctypes.c_int,
@@ -206,9 +210,34 @@ def batch(patterns, texts, max_distance=0, threads=1):
     return ok, dist[:n], slen[:n]
 
 
-def batch_workload(w, max_distance=0, threads=8, idx=None):
-    """Oracle over pairs of an ewfa.Workload (all, or the indices idx).
-    Returns (ok, dist, slen, streams) with streams a list of int8 arrays."""
+def batch_streams(patterns, texts, max_distance=0, threads=1):
+    """Oracle over lists of byte strings; returns (ok, dist, slen, streams)
+    with streams a list of int8 arrays."""
+    n = len(patterns)
+    pat = np.frombuffer(b"".join(patterns) + b"\0", dtype=np.uint8)
+    txt = np.frombuffer(b"".join(texts) + b"\0", dtype=np.uint8)
+    plen = np.array([len(x) for x in patterns] or [0], dtype=np.int32)
+    tlen = np.array([len(x) for x in texts] or [0], dtype=np.int32)
+    po = np.zeros(max(n, 1), dtype=np.int64)
+    to = np.zeros(max(n, 1), dtype=np.int64)
+    if n:
+        po[1:n] = np.cumsum(plen[:n])[:-1]
+        to[1:n] = np.cumsum(tlen[:n])[:-1]
+    sstride = int(max(8, (plen.astype(np.int64) + tlen).max(initial=0) + 8))
+    streams = np.zeros(max(n, 1) * sstride, dtype=np.int8)
+    dist = np.zeros(max(n, 1), dtype=np.int32)
+    slen = np.zeros(max(n, 1), dtype=np.int32)
+    ok = lib().or_batch_streams(pat.ctypes.data, po.ctypes.data, plen.ctypes.data, txt.ctypes.data,
+                                to.ctypes.data, tlen.ctypes.data, n, max_distance, threads,
+                                dist.ctypes.data, slen.ctypes.data, streams.ctypes.data, sstride)
+    out = [streams[i * sstride: i * sstride + min(int(slen[i]), sstride)] for i in range(n)]
+    return ok, dist[:n], slen[:n], out
+
+
+def batch_workload(w, max_distance=0, threads=8, idx=None, text=True):
+    """Oracle over pairs of an ewfa.Workload (all, or the indices idx), with
+    or without the CIGAR text step.  Returns (ok, dist, slen, streams) with
+    streams a list of int8 arrays."""
     if idx is None:
         idx = np.arange(w.n)
     idx = np.asarray(idx, dtype=np.int64)
@@ -222,9 +251,9 @@ def batch_workload(w, max_distance=0, threads=8, idx=None):
     streams = np.zeros(max(n, 1) * sstride, dtype=np.int8)
     dist = np.zeros(max(n, 1), dtype=np.int32)
     slen = np.zeros(max(n, 1), dtype=np.int32)
-    ok = lib().or_batch_streams(w.pat.ctypes.data, pat_off.ctypes.data, plen.ctypes.data,
-                                w.txt.ctypes.data, txt_off.ctypes.data, tlen.ctypes.data,
-                                n, max_distance, threads, dist.ctypes.data, slen.ctypes.data,
-                                streams.ctypes.data, sstride)
+    ok = lib().or_batch_text(w.pat.ctypes.data, pat_off.ctypes.data, plen.ctypes.data,
+                             w.txt.ctypes.data, txt_off.ctypes.data, tlen.ctypes.data,
+                             n, max_distance, threads, 1 if text else 0, dist.ctypes.data, slen.ctypes.data,
+                             streams.ctypes.data, sstride)
     out = [streams[i * sstride: i * sstride + min(int(slen[i]), sstride)] for i in range(n)]
     return ok, dist[:n], slen[:n], out

@@ -2,6 +2,9 @@
 (tolerance 0: the work is integer), queued jobs are merged into shared
 launches and overlap on the device, and an arena smaller than the offered load
 still admits and completes every job."""
+import os
+import time
+
 import numpy as np
 import pytest
 
@@ -56,6 +59,41 @@ def test_each_job_same_alone_and_in_the_mix():
     ok, d, sl, streams = orc.batch_workload(w)
     got = ewfa.streams_from(outs[3][0], w.pairs, outs[3][1])
     assert all(np.array_equal(a, b) for a, b in zip(got, streams))
+
+
+def test_served_mix_at_full_ranges_every_job_against_the_oracle():
+    """The served workload's ranges: jobs of 1..4,096 pairs, lengths
+    log-uniform 100..10,000 bp, 1..10% edits, arriving as a Poisson stream
+    while earlier jobs run.  Every job equals the oracle bit for bit and its
+    own run alone."""
+    rng = np.random.default_rng(55)
+    spec = ewfa.synth_spec(kind=1, seed=55, len_lo=100, len_hi=10_000, err_lo=0.01, err_hi=0.10)
+    sizes = [int(x) for x in rng.integers(1, 4097, 10)] + [4096, 1]
+    jobs, first = [], 0
+    for n in sizes:
+        jobs.append(ewfa.Workload(spec, n, first=first, threads=8))
+        first += n
+    gaps = rng.exponential(0.004, len(jobs))  # about 250 jobs/s
+    say(f"case serve c5 ranges: {len(jobs)} jobs, {first} pairs, Poisson arrivals, 2 slots")
+    with ewfa.Server(slots=2, max_batch_pairs=8192) as srv:
+        handles = []
+        for w, g in zip(jobs, gaps):
+            time.sleep(float(g))
+            handles.append(srv.submit(w.words, w.pairs, w.n, w.cigar_bytes))
+        outs = [h.wait() for h in handles]
+        st = srv.stats()
+    assert st["completed"] == len(jobs) and st["failed"] == 0
+    say("  served; checking every job against the oracle and alone")
+    threads = max(1, min(16, os.cpu_count() or 8))
+    for k, (w, (r, c)) in enumerate(zip(jobs, outs)):
+        ok, d, sl, streams = orc.batch_workload(w, threads=threads)
+        assert ok == w.n
+        got = ewfa.streams_from(r, w.pairs, c)
+        bad = [i for i in range(w.n) if int(r[i]["status"]) != ewfa.PAIR_OK or int(r[i]["distance"]) != int(d[i])
+               or not np.array_equal(got[i], streams[i])]
+        assert not bad, f"job {k}: {len(bad)} of {w.n} pairs differ from the oracle (first {bad[0]})"
+        r0, c0 = alone(w)
+        assert np.array_equal(r0, r), f"job {k}: results differ from the job run alone"
 
 
 def footprint(w):

@@ -134,6 +134,74 @@ def test_reader_errors(tmp_path):
     assert rc == ewfa.ERR_IO
 
 
+def test_reader_error_message_is_the_references(tmp_path):
+    # utils/sequence_reader.cu:157-158: "Invalid sequence format on line %d.\n    %s\n"
+    path = os.path.join(tmp_path, "bad.txt")
+    with open(path, "wb") as f:
+        f.write(b">ACGT\n<ACGT\n>AC\nTT\n")
+    rc, _ = read_product(path, 10, 2)
+    assert rc == ewfa.ERR_FORMAT
+    assert ewfa.lib().ewfa_last_error() == b"Invalid sequence format on line 3.\n    TT\n\n"
+
+
+@pytest.mark.parametrize("threads,n", [(1, 9000), (3, 9000), (8, 9000), (8, 4000), (5, 20000)])
+def test_reader_across_scan_blocks(tmp_path, threads, n):
+    # about 20 MB: line starts are found in 8 MiB blocks, one thread a block;
+    # lines straddle block ends, the file ends without a newline, and n may
+    # stop before the file does or ask for more pairs than it holds
+    rng = np.random.default_rng(7)
+    lines = []
+    for i in range(2 * 9000):
+        L = int(rng.integers(0, 2400))
+        seq = bytes(rng.choice(list(b"ACGTN"), L).astype(np.uint8))
+        lines.append((b">" if i % 2 == 0 else b"<") + seq)
+    data = b"\n".join(lines)  # no final newline
+    path = os.path.join(tmp_path, "big.txt")
+    with open(path, "wb") as f:
+        f.write(data)
+    f_ = ewfa.PairsFile()
+    rc = ewfa.lib().ewfa_read_pairs(path.encode(), 2500, n, threads, ctypes.byref(f_))
+    assert rc == ewfa.OK
+    got_words = np.ctypeslib.as_array(f_.words, shape=(f_.n_words,)).copy()
+    pairs = np.ctypeslib.as_array((ctypes.c_uint8 * (40 * n)).from_address(f_.pairs)).view(ewfa.PAIR_DTYPE).copy()
+    ewfa.lib().ewfa_free_pairs(ctypes.byref(f_))
+    orc_rc, opairs, _, _ = orc.read_pairs(path, 2500, n)
+    assert orc_rc == 1
+    for i in range(0, n, 97):
+        p, t = opairs[i]
+        assert int(pairs[i]["plen"]) == len(p) and int(pairs[i]["tlen"]) == len(t), i
+        for seq, off in ((p, int(pairs[i]["pattern_off"])), (t, int(pairs[i]["text_off"]))):
+            want = orc.pack_u64(seq)
+            assert np.array_equal(got_words[off:off + len(want)], want), i
+
+
+# ---------------------------------------------------------------------------
+# launch policy files
+
+def test_policy_builtin_and_files(tmp_path):
+    builtin = ewfa.read_policy(None)
+    assert 1 <= len(builtin) <= 8 and builtin[-1][0] == 0xFFFFFFFF
+    good = os.path.join(tmp_path, "good.policy")
+    with open(good, "w") as f:
+        f.write("# len_hi waves dcap bt_lds grab\n300 1 63 1 16\n5000 2 512 0 1\n9000 4 700 0 1\n")
+    assert ewfa.read_policy(good) == [(300, 1, 63, 1, 16), (5000, 2, 512, 0, 1), (0xFFFFFFFF, 4, 700, 0, 1)]
+    # a plan holds 8 passes: a ninth bucket would never be planned, so the
+    # file is refused rather than its longest pairs failing
+    nine = os.path.join(tmp_path, "nine.policy")
+    with open(nine, "w") as f:
+        for i in range(9):
+            f.write(f"{(i + 1) * 100} 1 63 1 4\n")
+    with pytest.raises(ewfa.EwfaError):
+        ewfa.read_policy(nine)
+    eight = os.path.join(tmp_path, "eight.policy")
+    with open(eight, "w") as f:
+        for i in range(8):
+            f.write(f"{(i + 1) * 100} 1 63 1 4\n")
+    assert len(ewfa.read_policy(eight)) == 8
+    with pytest.raises(ewfa.EwfaError):
+        ewfa.read_policy(os.path.join(tmp_path, "missing.policy"))
+
+
 # ---------------------------------------------------------------------------
 # generator
 
