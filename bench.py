@@ -348,6 +348,82 @@ def leg_served(args, threads, batched_rate):
     return out
 
 
+def write_pair_file(w, path):
+    """The workload as a '>'/'<' pair file (README.md:58-69)."""
+    import numpy as np
+    with open(path, "wb") as f:
+        buf = []
+        for i in range(w.n):
+            buf.append(b">" + w.pattern(i) + b"\n<" + w.text(i) + b"\n")
+            if len(buf) == 4096:
+                f.write(b"".join(buf))
+                buf = []
+        f.write(b"".join(buf))
+    return os.path.getsize(path)
+
+
+def leg_cli(args, w, threads):
+    """wfe.aligner on the 1,000,000-pair file: the reference's own figure
+    ("Aligned executed. Wall time", src/aligner.cu:233) with and without -p,
+    a sample of the printed lines against the oracle's text, and the pair-file
+    ingestion (file mapped, lines found, sequences packed) timed apart."""
+    import ctypes
+    import re
+    import subprocess
+    import tempfile
+    import numpy as np
+    import ewfa
+    import oracle_py as orc
+    tmp = tempfile.mkdtemp(prefix="ewfa_cli_")
+    path = os.path.join(tmp, "c2_pairs.txt")
+    t0 = time.time()
+    size = write_pair_file(w, path)
+    log(f"cli: wrote {w.n} pairs ({size / 1e9:.2f} GB) in {time.time() - t0:.1f}s")
+    out = {"pairs": w.n, "file_bytes": size, "batch": args.batch}
+    f = ewfa.PairsFile()
+    t0 = time.perf_counter()
+    rc = ewfa.lib().ewfa_read_pairs(path.encode(), 1000, w.n, threads, ctypes.byref(f))
+    out["ingest_s"] = round(time.perf_counter() - t0, 3)
+    out["ingest_scan_s"] = round(f.scan_s, 3)
+    out["ingest_pack_s"] = round(f.pack_s, 3)
+    ewfa.lib().ewfa_free_pairs(ctypes.byref(f))
+    assert rc == 0
+    cmd = [ewfa.ALIGNER_PATH, "-f", path, "-n", str(w.n), "-l", "1000", "-b", str(args.batch)]
+    wall = lambda text: float(re.search(r"Aligned executed\. Wall time ([0-9.]+)s", text).group(1))
+    for key, extra in (("no_print", []), ("print", ["-p"])):
+        sink = os.path.join(tmp, f"out_{key}.txt")
+        t0 = time.perf_counter()
+        with open(sink, "wb") as so:
+            r = subprocess.run(cmd + extra, stdout=so, stderr=subprocess.PIPE, timeout=300)
+        proc = time.perf_counter() - t0
+        assert r.returncode == 0, r.stderr.decode()[-500:]
+        with open(sink, "rb") as so:
+            so.seek(max(0, os.path.getsize(sink) - 200))
+            tail = so.read().decode(errors="replace")
+        s = wall(tail)
+        out[key] = {"wall_s": s, "value": round(w.n / s, 1), "process_s": round(proc, 3)}
+        log(f"cli {key}: wall {s:.3f}s -> {w.n / s / 1e6:.2f} M aln/s (process {proc:.2f}s)")
+        if key == "print":
+            # every 1000th line against the oracle's text
+            want = {}
+            idx = np.arange(0, w.n, 1000)
+            ok, od, osl, ost = orc.batch_workload(w, idx=idx, threads=threads)
+            for j, i in enumerate(idx):
+                want[int(i)] = f"{int(i)}: {orc.text(ost[j])}"
+            bad = 0
+            with open(sink, "rb") as so:
+                for line in so:
+                    m = re.match(rb"^(\d+): ", line)
+                    if m and int(m.group(1)) in want:
+                        bad += line.decode().rstrip("\n") != want.pop(int(m.group(1)))
+            out[key]["lines_checked"] = len(idx)
+            out[key]["mismatches"] = bad + len(want)
+        os.unlink(sink)
+    os.unlink(path)
+    os.rmdir(tmp)
+    return out
+
+
 def oracle_check(w, res, cig, count, threads):
     """Distances and streams of `count` evenly spaced pairs against the oracle."""
     import numpy as np
@@ -425,7 +501,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--batch", type=int, default=100_000)
-    ap.add_argument("--legs", default="c2,c3,c4,served", help="comma list of c2,c3,c4,served")
+    ap.add_argument("--legs", default="c2,cli,c3,c4,served", help="comma list of c2,cli,c3,c4,served")
     ap.add_argument("--c3-pairs", type=int, default=500_000)
     ap.add_argument("--c3-steps", type=int, default=3)
     ap.add_argument("--c4-pairs", type=int, default=20_000)
@@ -433,16 +509,18 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-pool", type=int, default=384)
     ap.add_argument("--serve-ring", type=int, default=6, help="output sets per pool job")
-    ap.add_argument("--serve-slots", type=int, default=1)
-    ap.add_argument("--serve-batch", type=int, default=524288, help="max pairs per launch")
+    ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-batch", type=int, default=131072, help="max pairs per launch")
     ap.add_argument("--serve-batch-bytes", type=int, default=2048, help="max CIGAR MiB per launch")
     ap.add_argument("--serve-arena", type=int, default=24, help="GiB")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--c4-check", type=int, default=32, help="long-read pairs checked against the oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=3000, help="pairs checked against the oracle")
-    ap.add_argument("--profile-run", choices=["c2", "c3", "c4"],
-                    help="two eager persistent passes over one workload only (for rocprofv3)")
+    ap.add_argument("--profile-run", choices=["c2", "c3", "c4", "served"],
+                    help="the bench's timed loop over one workload only (for rocprofv3)")
+    ap.add_argument("--eager", action="store_true",
+                    help="with --profile-run: launch each kernel eagerly instead of replaying a graph")
     ap.add_argument("--profile-copy", action="store_true",
                     help="one 1 GiB device copy only (HBM counter calibration)")
     args = ap.parse_args()
@@ -468,22 +546,33 @@ def main():
         log("copied 1 GiB")
         return
 
+    if args.profile_run == "served":
+        # the served leg alone (after the c3 leg it is offered against)
+        c3 = leg_c3(args, threads)
+        served = leg_served(args, threads, c3["value"])
+        log(json.dumps({k: v for k, v in served.items() if k not in ("w", "res", "cig")}))
+        return
+
     if args.profile_run:
+        # the bench's own timed loop for one workload (graph replay with the
+        # timing events, --warmup then --steps calls), so that a profiler's
+        # per-kernel averages compare with the bench's kernels_ms
         name = args.profile_run
-        n = {"c2": args.pairs, "c3": args.c3_pairs, "c4": min(args.c4_pairs, 4000)}[name]
+        n = {"c2": args.pairs, "c3": args.c3_pairs, "c4": args.c4_pairs}[name]
         cap = 1004 if name == "c2" else 0
         w = ewfa.Workload(ewfa.synth_spec(**ewfa.CONFIGS[name]), n, cigar_cap=cap, threads=threads)
         r = Resident(w)
         pol = {"c2": POLICY_C2, "c3": POLICY_C3, "c4": None}[name]
-        with ewfa.Context(mode=ewfa.MODE_PERSISTENT, flags=ewfa.FLAG_NO_GRAPH, policy_path=pol) as ctx:
-            for _ in range(2):
-                r.run(ctx)
-                ewfa.synchronize()
-        log(f"profile run {name} ({n} pairs) done")
+        flags = ewfa.FLAG_TIMING | (ewfa.FLAG_NO_GRAPH if args.eager else 0)
+        with ewfa.Context(mode=ewfa.MODE_PERSISTENT, flags=flags, policy_path=pol) as ctx:
+            dt, a_ms, e_ms = timed(r.run, ctx, args.steps, args.warmup)
+        log(f"profile run {name}: {n} pairs, {args.steps} steps in {dt:.3f}s -> {args.steps * n / dt / 1e6:.3f} M aln/s; "
+            f"align {np.mean(a_ms):.3f} ms, emit {np.mean(e_ms):.3f} ms per step (events)")
         return
 
     legs = set(args.legs.split(","))
     c2 = leg_c2(args, threads)
+    cli = leg_cli(args, c2["w"], threads) if "cli" in legs else None
     c3 = leg_c3(args, threads) if "c3" in legs else None
     c4 = leg_c4(args, threads) if "c4" in legs else None
     served = leg_served(args, threads, c3["value"]) if ("served" in legs and c3) else None
@@ -572,6 +661,8 @@ def main():
                      "cells_per_s": round(c4["cells"] / c4["seconds"], 1)}
     if served:
         out["served"] = {k: v for k, v in served.items() if k not in ("w", "res", "cig")}
+    if cli:
+        out["cli"] = cli
     if c1_cpu:
         out["cpu_c1"] = dict(c1_cpu, unit="alignments/s", sample="c1: 10,000 x 150 bp, 2% edits, whole")
     out["oracle_checks"] = {k: {"pairs": v[0], "mismatches": v[1]} for k, v in checks.items()}

@@ -70,34 +70,47 @@ struct Batch {
     std::vector<ewfa_pair_t> pairs;
 };
 
-void format_batch(const Params& p, const Batch& b) {
-    std::string out;
-    std::vector<char> text;
-    if (p.max_distance > 0) {
-        // the reference prints these from the device before the batch's lines
-        for (size_t i = 0; i < b.count; ++i)
-            if (b.results[i].status == EWFA_PAIR_CAPPED || b.results[i].status == EWFA_PAIR_UNDEFINED) {
-                char line[96];
-                snprintf(line, sizeof line, "Max distance reached at alignment %zu\n", i);
-                out += line;
-            }
-    }
+// Host threads that format one batch's text (shared by the -t workers).
+int g_format_threads = 1;
+
+// Writes "<index>: " at out, returns its length.
+size_t put_index(char* out, size_t idx) {
+    char tmp[24];
+    size_t n = 0;
+    do {
+        tmp[n++] = (char)('0' + idx % 10);
+        idx /= 10;
+    } while (idx);
+    for (size_t i = 0; i < n; ++i) out[i] = tmp[n - 1 - i];
+    out[n] = ':';
+    out[n + 1] = ' ';
+    return n + 2;
+}
+
+// Pairs [lo, hi) of a batch: their "<index>: <text>" lines appended to out
+// (the reference's print loop, src/wavefront.cu:293-300) and the number whose
+// stream passes the reference's check (src/wavefront.cu:302-320).
+int format_range(const Params& p, const Batch& b, size_t lo, size_t hi, std::string& out) {
     int correct = 0;
-    for (size_t i = 0; i < b.count; ++i) {
+    for (size_t i = lo; i < hi; ++i) {
         const ewfa_pair_t& pr = b.pairs[i];
         const ewfa_result_t& r = b.results[i];
         const int8_t* s = b.cigars.data() + pr.cigar_off;
-        size_t len = std::min<size_t>((size_t)std::max(r.cigar_len, 0), pr.cigar_cap);
+        const size_t len = std::min<size_t>((size_t)std::max(r.cigar_len, 0), pr.cigar_cap);
         if (p.print) {
-            text.resize(8 * len + 16);
-            int n = p.standard ? ewfa_cigar_text_standard(s, len, text.data(), text.size())
-                               : ewfa_cigar_text(s, len, text.data(), text.size());
-            if (n < 0) n = 0, text[0] = 0;
-            char head[32];
-            snprintf(head, sizeof head, "%d: ", (int)(b.first + i));
-            out += head;
-            out.append(text.data(), (size_t)n);
-            out += '\n';
+            // a stream byte prints as at most 4 characters ("68MD"); standard
+            // runs as at most 11 ("4294967295=")
+            const size_t room = 24 + (p.standard ? 12 : 4) * len + 2;
+            const size_t at = out.size();
+            out.resize(at + room);
+            char* o = &out[at];
+            size_t k = put_index(o, b.first + i);
+            int n = p.standard ? ewfa_cigar_text_standard(s, len, o + k, room - k - 1)
+                               : ewfa_cigar_text(s, len, o + k, room - k - 1);
+            if (n < 0) n = 0;
+            k += (size_t)n;
+            o[k++] = '\n';
+            out.resize(at + k);
         }
         if (p.check) {
             const size_t g = b.first + i;
@@ -106,9 +119,45 @@ void format_batch(const Params& p, const Batch& b) {
             if (ewfa_check_cigar(pat, pr.plen, txt, pr.tlen, s, len) == 1) ++correct;
         }
     }
+    return correct;
+}
+
+// The batch's output: its lines formatted on g_format_threads threads over
+// contiguous ranges, then written in index order as whole lines.
+void format_batch(const Params& p, const Batch& b) {
+    std::string head;
+    if (p.max_distance > 0) {
+        // the reference prints these from the device before the batch's lines
+        for (size_t i = 0; i < b.count; ++i)
+            if (b.results[i].status == EWFA_PAIR_CAPPED || b.results[i].status == EWFA_PAIR_UNDEFINED) {
+                char line[96];
+                snprintf(line, sizeof line, "Max distance reached at alignment %zu\n", i);
+                head += line;
+            }
+    }
+    const size_t nt = (p.print || p.check) ? std::max<size_t>(1, std::min<size_t>((size_t)g_format_threads,
+                                                                                  b.count / 2048 + 1))
+                                           : 0;
+    std::vector<std::string> outs(nt);
+    std::vector<int> correct(nt, 0);
+    if (nt) {
+        const size_t per = (b.count + nt - 1) / nt;
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; ++t)
+            pool.emplace_back([&, t] {
+                correct[t] = format_range(p, b, std::min(b.count, t * per), std::min(b.count, (t + 1) * per), outs[t]);
+            });
+        correct[0] = format_range(p, b, 0, std::min(b.count, per), outs[0]);
+        for (std::thread& th : pool) th.join();
+    }
     std::lock_guard<std::mutex> lk(g_out_mutex);
-    fwrite(out.data(), 1, out.size(), stdout);
-    if (p.check) fprintf(stderr, "Correct alignments: %d/%zu\n", correct, b.count);
+    fwrite(head.data(), 1, head.size(), stdout);
+    for (const std::string& o : outs) fwrite(o.data(), 1, o.size(), stdout);
+    if (p.check) {
+        int c = 0;
+        for (int x : correct) c += x;
+        fprintf(stderr, "Correct alignments: %d/%zu\n", c, b.count);
+    }
 }
 
 void* worker(void* arg) {
@@ -281,6 +330,9 @@ int main(int argc, char** argv) {
         fflush(stdout);
         exit(1);
     }
+
+    // formatting threads per worker: the host's cores shared by the -t workers
+    g_format_threads = (int)std::max(1u, std::min(hw, 16u) / std::max(1u, num_threads));
 
     struct timespec t0, t1;
     clock_gettime(CLOCK_REALTIME, &t0);

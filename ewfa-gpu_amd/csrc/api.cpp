@@ -84,9 +84,16 @@ struct ewfa_ctx {
     ewfa::Policy policy;
     hipStream_t stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_a0 = nullptr, ev_a1 = nullptr, ev_e1 = nullptr;
+    // the persistent plan's bucket passes spread over `fanout` streams: the
+    // context's own and aux[0..fanout-2], forked from and joined back into it
+    // with ev_fork / ev_join, so that one pass's tail runs beside the next
+    static constexpr int MAX_FANOUT = 3;
+    int fanout = MAX_FANOUT;
+    hipStream_t aux[MAX_FANOUT - 1] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[MAX_FANOUT - 1] = {};
     int num_cus = 0;
     size_t lds_max = 0;
-    DevBuf ops, bt, ctrl, list_a, list_b, buckets, runs, list_fb, bound;
+    DevBuf ops, bt, ctrl, list_a, list_b, buckets, runs, list_fb, bound, gslot;
     uint32_t* host_ctrl = nullptr;  // pinned mirror of the control words
     float align_ms = 0.f, emit_ms = 0.f;
     bool timing = false;
@@ -110,6 +117,9 @@ struct ewfa_ctx {
     // op scratch supplied by the caller for one call (the job server's arena)
     uint64_t* ops_ext = nullptr;
     size_t ops_ext_bytes = 0;
+    // all of one call's scratch in caller memory (the allocation-free entry
+    // point): when set, no context buffer is used or grown
+    const ewfa::Scratch* ext = nullptr;
     // host pipeline buffers
     DevBuf h_words, h_pairs, h_results, h_cigars;
     // CIGAR compaction: dense device copy, block sums, pinned host staging
@@ -171,7 +181,31 @@ struct Pass {
     size_t lds;
     uint32_t grid;
     bool persistent;
+    size_t slot_off;  // byte offset of the pass's slots in the backtrace (block
+                      // passes) or run-row (register passes) region
+    int gslot;          // 1: the block's slot (sequence copies, offsets) in HBM
+    size_t gslot_bytes; // bytes of one such slot
+    size_t gslot_off;   // byte offset of the pass's HBM slots
 };
+
+// Passes whose wavefront LDS cannot hold keep their slots in HBM.  Their
+// backtrace slot grows as the square of the budget, so the budget stops where
+// one block's backtrace slot reaches GSLOT_BT_MAX bytes (about 131,000 levels),
+// and a pass gets at most as many blocks as fit GSLOT_BUDGET.
+constexpr size_t GSLOT_BT_MAX = (size_t)4 << 30;
+constexpr size_t GSLOT_BUDGET = (size_t)2 << 30;
+
+int gslot_max_dcap() {
+    static const int most = [] {
+        int lo = 1, hi = 1 << 20;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (16u * (size_t)ewfa::bt_chunks((uint32_t)mid) <= GSLOT_BT_MAX) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    }();
+    return most;
+}
 
 // 64-bit words of one sequence slot in LDS: the data plus two pad words.
 int seq_words_for(uint32_t max_len) { return (int)(max_len / 32u) + 3; }
@@ -206,6 +240,19 @@ void choose_off16(Pass& p, uint32_t max_len) {
 }
 
 int finish_pass(ewfa_ctx* c, Pass& p, uint32_t items) {
+    if (p.gslot) {
+        // slot in HBM, no dynamic LDS; blocks bounded by occupancy and memory
+        p.lds = 0;
+        p.gslot_bytes = (ewfa::lds_bytes(p.seq_words, p.dcap, 0, 0) + 255) & ~(size_t)255;
+        const int bpc = std::max(1, std::min(ewfa::align_blocks_per_cu(ewfa::GSLOT_WAVES, 0, 4),
+                                             ewfa::align_blocks_per_cu(ewfa::GSLOT_WAVES, 0, 5)));
+        uint64_t g = (uint64_t)bpc * (uint64_t)c->num_cus;
+        if (items > 0 && g > items) g = items;
+        const size_t per_block = p.gslot_bytes + 16u * (size_t)ewfa::bt_chunks((uint32_t)p.dcap);
+        g = std::min<uint64_t>(g, std::max<size_t>(GSLOT_BUDGET / per_block, 1));
+        p.grid = (uint32_t)std::max<uint64_t>(g, 1);
+        return EWFA_OK;
+    }
     p.lds = p.reg ? ewfa::lds_bytes_reg(p.seq_words, p.dcap, p.reg)
                   : ewfa::lds_bytes(p.seq_words, p.dcap, p.bt_lds, p.off16);
     if (p.lds > lds_budget(c))
@@ -236,7 +283,14 @@ int pass_from_bucket(ewfa_ctx* c, const ewfa::BucketPolicy& b, uint32_t blen, ui
     p.dcap = b.dcap;
     // the bucket's dcap is a request; shrink it to what LDS holds
     const int most = max_dcap_for(p.seq_words, p.bt_lds, lds_budget(c));
-    if (most < 0) return fail(EWFA_ERR_RANGE, "sequences of %u bases do not fit in LDS", blen);
+    if (most < 0) {
+        // sequences too long for LDS: the block's slot goes to HBM
+        p.gslot = 1;
+        p.waves = ewfa::GSLOT_WAVES;
+        p.bt_lds = 0;
+        p.dcap = std::max(1, std::min({p.dcap, (int)blen + 1, gslot_max_dcap()}));
+        return finish_pass(c, p, items);
+    }
     if (p.dcap > most) p.dcap = most;
     if (p.dcap < 1) p.dcap = 1;
     // one wave with the backtrace in LDS: offsets in registers when they fit
@@ -291,7 +345,47 @@ struct Plan {
     Pass pass[ewfa::MAX_BUCKETS];
     uint32_t blen[ewfa::MAX_BUCKETS];
     Chain chain;
+    // streams the bucket passes are spread over (1: one after another); with
+    // more, each bucket pass has slots of its own
+    int fanout = 1;
+    size_t bt_bytes = 0, runs_bytes = 0;  // regions the plan's slots need
+    size_t gslot_bytes = 0;               // HBM slots of the passes beyond LDS
 };
+
+// Bytes of one block's slot in the backtrace region (block passes with the
+// backtrace in HBM) or the run-row region (multi-chunk register passes).
+size_t slot_bytes(const Pass& p);
+
+// Gives each pass its slots: bucket passes that may run side by side get
+// disjoint ranges, the deferral chain (after them) starts at 0 again.
+void assign_slots(Plan& pl) {
+    size_t bt = 0, runs = 0;
+    for (int b = 0; b < pl.nb; ++b) {
+        Pass& p = pl.pass[b];
+        size_t& at = p.reg ? runs : bt;
+        p.slot_off = pl.fanout > 1 ? at : 0;
+        const size_t need = p.slot_off + slot_bytes(p) * p.grid;
+        if (pl.fanout > 1) at = need;
+        else at = std::max(at, need);
+    }
+    size_t gs = 0;
+    for (int b = 0; b < pl.nb; ++b) {
+        Pass& p = pl.pass[b];
+        if (!p.gslot) continue;
+        p.gslot_off = pl.fanout > 1 ? gs : 0;
+        gs = std::max(gs, p.gslot_off + p.gslot_bytes * p.grid);
+    }
+    for (int k = 0; k < pl.chain.np; ++k) {
+        Pass& p = pl.chain.pass[k];
+        p.slot_off = 0;
+        p.gslot_off = 0;
+        bt = std::max(bt, slot_bytes(p) * p.grid);
+        if (p.gslot) gs = std::max(gs, p.gslot_bytes * p.grid);
+    }
+    pl.bt_bytes = bt;
+    pl.runs_bytes = runs;
+    pl.gslot_bytes = gs;
+}
 
 int next_pass(ewfa_ctx* c, uint32_t max_len, int prev_dcap, uint32_t items, Pass& p);
 
@@ -326,6 +420,8 @@ int plan_persistent(ewfa_ctx* c, uint32_t max_len, uint32_t n, Plan& pl) {
     int dcap_min = pl.pass[0].dcap;
     for (int b = 1; b < pl.nb; ++b) dcap_min = std::min(dcap_min, pl.pass[b].dcap);
     plan_chain(c, max_len, dcap_min, n, pl.chain);
+    pl.fanout = std::min(pl.nb, c->fanout);
+    assign_slots(pl);
     return EWFA_OK;
 }
 
@@ -337,10 +433,18 @@ int next_pass(ewfa_ctx* c, uint32_t max_len, int prev_dcap, uint32_t items, Pass
     p.grab = 1;
     p.waves = 4;
     const int most_global = max_dcap_for(p.seq_words, 0, lds_budget(c));
-    if (most_global < 0) return fail(EWFA_ERR_RANGE, "sequences of %u bases do not fit in LDS", max_len);
     int want = std::max(prev_dcap * 4, 256);
-    const int dmax = (int)max_len + 1;  // no pair needs more than max(plen,tlen)
+    // no pair needs more than max(plen, tlen) levels
+    const int dmax = std::min((int)max_len + 1, gslot_max_dcap());
     if (want > dmax) want = dmax;
+    if (most_global <= prev_dcap) {
+        // LDS holds no larger wavefront: the block's slot goes to HBM
+        if (want <= prev_dcap) return fail(EWFA_ERR_RANGE, "no larger wavefront budget available");
+        p.gslot = 1;
+        p.waves = ewfa::GSLOT_WAVES;
+        p.dcap = want;
+        return finish_pass(c, p, items);
+    }
     if (want > most_global) want = most_global;
     if (want <= prev_dcap) return fail(EWFA_ERR_RANGE, "no larger wavefront budget available");
     p.dcap = want;
@@ -351,7 +455,28 @@ int next_pass(ewfa_ctx* c, uint32_t max_len, int prev_dcap, uint32_t items, Pass
     return finish_pass(c, p, items);
 }
 
-uint64_t* ops_base(ewfa_ctx* c) { return c->ops_ext ? c->ops_ext : c->ops.as<uint64_t>(); }
+uint64_t* ops_base(ewfa_ctx* c) {
+    if (c->ext) return c->ext->ops;
+    return c->ops_ext ? c->ops_ext : c->ops.as<uint64_t>();
+}
+
+// The scratch a call's kernels use: the external one of an allocation-free
+// call, else the context's own buffers.
+ewfa::Scratch scratch_of(ewfa_ctx* c) {
+    if (c->ext) return *c->ext;
+    ewfa::Scratch x{};
+    x.ops = ops_base(c);
+    x.ctrl = c->ctrl.as<uint32_t>();
+    x.list_a = c->list_a.as<uint32_t>();
+    x.list_b = c->list_b.as<uint32_t>();
+    x.list_fb = c->list_fb.as<uint32_t>();
+    x.buckets = c->buckets.as<uint32_t>();
+    x.bound = c->bound.as<uint32_t>();
+    x.bt = c->bt.as<uint64_t>();
+    x.runs = c->runs.as<uint8_t>();
+    x.gslot = c->gslot.as<uint64_t>();
+    return x;
+}
 
 int reserve_scratch(ewfa_ctx* c, size_t n, uint32_t max_len) {
     const uint32_t ops_stride = ewfa::ops_words(max_len + 1);
@@ -375,16 +500,23 @@ uint32_t runs_slot_of(const Pass& p) {
     return ewfa::runs_slot_bytes((uint32_t)std::min(p.dcap, 32 * p.reg - 1));
 }
 
+size_t slot_bytes(const Pass& p) {
+    if (p.reg) return runs_slot_of(p);
+    if (p.bt_lds) return 0;
+    return 16u * (size_t)ewfa::bt_chunks((uint32_t)p.dcap);
+}
+
 int reserve_bt(ewfa_ctx* c, const Pass& p) {
-    if (p.reg) return c->runs.reserve(c, (size_t)runs_slot_of(p) * p.grid);
+    if (p.reg) return c->runs.reserve(c, p.slot_off + (size_t)runs_slot_of(p) * p.grid);
     if (p.bt_lds) return EWFA_OK;
-    const size_t slot = 2u * (size_t)ewfa::bt_chunks((uint32_t)p.dcap);
-    return c->bt.reserve(c, slot * p.grid * 8);
+    if (p.gslot) CHK(c->gslot.reserve(c, p.gslot_off + p.gslot_bytes * p.grid));
+    return c->bt.reserve(c, p.slot_off + slot_bytes(p) * p.grid);
 }
 
 int reserve_plan(ewfa_ctx* c, const Plan& pl, size_t n) {
-    for (int b = 0; b < pl.nb; ++b) CHK(reserve_bt(c, pl.pass[b]));
-    for (int k = 0; k < pl.chain.np; ++k) CHK(reserve_bt(c, pl.chain.pass[k]));
+    CHK(c->runs.reserve(c, pl.runs_bytes));
+    CHK(c->bt.reserve(c, pl.bt_bytes));
+    CHK(c->gslot.reserve(c, pl.gslot_bytes));
     if (pl.nb > 1) CHK(c->buckets.reserve(c, (size_t)pl.nb * std::max<size_t>(n, 1) * 4));
     CHK(c->bound.reserve(c, std::max<size_t>(n, 1) * 4));
     return EWFA_OK;
@@ -411,7 +543,7 @@ int enqueue_bound(ewfa_ctx* c, ewfa::AlignArgs& a, const Pass& p, uint32_t* queu
     if (p.reg || p.dcap < BOUND_MIN_DCAP || lds > 65536 || n == 0) return EWFA_OK;
     ewfa::AlignArgs b = a;
     b.queue = queue;
-    b.bound = c->bound.as<uint32_t>();
+    b.bound = scratch_of(c).bound;
     const size_t per_cu = std::max<size_t>(1, std::min<size_t>(16, lds_budget(c) / lds));
     const uint32_t grid = (uint32_t)std::min<size_t>((size_t)c->num_cus * per_cu, n);
     HIPCHK(ewfa::launch_bound(b, grid, lds, s));
@@ -437,9 +569,12 @@ void fill_pass_args(ewfa::AlignArgs& a, const Pass& p, ewfa_ctx* c) {
     a.bt_lds = p.bt_lds;
     a.seq_words = p.seq_words;
     a.grab = (uint32_t)p.grab;
-    a.bt_global = c->bt.as<uint64_t>();
+    const ewfa::Scratch sx = scratch_of(c);
+    a.bt_global = sx.bt + p.slot_off / 8;
     a.bt_slot_words = 2u * (uint64_t)ewfa::bt_chunks((uint32_t)p.dcap);
     a.off16 = p.off16;
+    a.gslot = p.gslot ? sx.gslot + p.gslot_off / 8 : nullptr;
+    a.gslot_words = p.gslot ? p.gslot_bytes / 8 : 0;
 }
 
 // The emit kernel for a pass: sequences staged in LDS when they fit (the
@@ -480,7 +615,8 @@ int emit(ewfa_ctx* c, const ewfa::EmitArgs& e, uint32_t items, uint32_t max_len,
 int enqueue_main(ewfa_ctx* c, const Pass& p, const uint64_t* words, const ewfa_pair_t* pairs,
                  size_t first, uint32_t count, uint32_t max_len, ewfa_result_t* results,
                  int8_t* cigars, hipStream_t s, bool timing) {
-    uint32_t* ctrl = c->ctrl.as<uint32_t>();
+    const ewfa::Scratch sx = scratch_of(c);
+    uint32_t* ctrl = sx.ctrl;
     ewfa::AlignArgs a = base_args(c, words, pairs, results, max_len);
     fill_pass_args(a, p, c);
     // batch-local view: shift the pair/result bases, the ops scratch is per pair
@@ -491,7 +627,7 @@ int enqueue_main(ewfa_ctx* c, const Pass& p, const uint64_t* words, const ewfa_p
     a.count = count;
     a.count_ptr = nullptr;
     a.queue = p.persistent ? ctrl + CTRL_QUEUE : nullptr;
-    a.defer_list = c->list_a.as<uint32_t>();
+    a.defer_list = sx.list_a;
     a.defer_count = ctrl + CTRL_DEFER_A;
     HIPCHK(ewfa::launch_fill_u32(ctrl, 0u, 3, s));
     if (timing) HIPCHK(hipEventRecord(c->ev_a0, s));
@@ -523,8 +659,9 @@ int enqueue_main(ewfa_ctx* c, const Pass& p, const uint64_t* words, const ewfa_p
 // resolve.
 int enqueue_chain(ewfa_ctx* c, const Chain& ch, const uint64_t* words, const ewfa_pair_t* pairs, uint32_t n,
                   uint32_t max_len, ewfa_result_t* results, int8_t* cigars, hipStream_t s) {
-    uint32_t* ctrl = c->ctrl.as<uint32_t>();
-    uint32_t* lists[2] = {c->list_a.as<uint32_t>(), c->list_b.as<uint32_t>()};
+    const ewfa::Scratch sx = scratch_of(c);
+    uint32_t* ctrl = sx.ctrl;
+    uint32_t* lists[2] = {sx.list_a, sx.list_b};
     const int ctr[2] = {CTRL_DEFER_A, CTRL_DEFER_B};
     int cur = 0;
     for (int k = 0; k < ch.np; ++k) {
@@ -559,62 +696,76 @@ int enqueue_chain(ewfa_ctx* c, const Chain& ch, const uint64_t* words, const ewf
     return EWFA_OK;
 }
 
-// phase: PLAN_ALL, or PLAN_ALIGN (bucketing and align passes) then PLAN_EMIT
-// (emits and the count's copy) as two graphs timed between their launches.
+// phase: PLAN_ALL, or PLAN_ALIGN (bucketing, the bucket passes and the emits
+// of the block buckets) then PLAN_EMIT (the fused passes' fallback emit and
+// the deferral chain) as two graphs timed between their launches.
 enum { PLAN_ALL = 3, PLAN_ALIGN = 1, PLAN_EMIT = 2 };
 
 int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_pair_t* pairs, uint32_t n,
                  uint32_t max_len, ewfa_result_t* results, int8_t* cigars, hipStream_t s, bool timing,
                  int phase = PLAN_ALL) {
-    uint32_t* ctrl = c->ctrl.as<uint32_t>();
-    uint32_t* lists = c->buckets.as<uint32_t>();
+    const ewfa::Scratch sx = scratch_of(c);
+    uint32_t* ctrl = sx.ctrl;
+    uint32_t* lists = sx.buckets;
     const bool bucketed = pl.nb > 1;
+    const int F = std::max(1, std::min(pl.fanout, ewfa_ctx::MAX_FANOUT));
+    hipStream_t st[ewfa_ctx::MAX_FANOUT] = {s};
+    for (int i = 1; i < F; ++i) st[i] = c->aux[i - 1];
+    bool fused = false;
+    for (int b = 0; b < pl.nb; ++b) fused = fused || pl.pass[b].reg;
     if (phase & PLAN_ALIGN) {
         HIPCHK(ewfa::launch_fill_u32(ctrl, 0u, CTRL_BOUNDQ + ewfa::MAX_BUCKETS, s));
         if (bucketed) HIPCHK(ewfa::launch_bucketize(pairs, n, pl.table, lists, ctrl + CTRL_BCOUNT, s));
         if (timing) HIPCHK(hipEventRecord(c->ev_a0, s));
-        for (int b = pl.nb - 1; b >= 0; --b) {
+        if (F > 1) {
+            HIPCHK(hipEventRecord(c->ev_fork, s));
+            for (int i = 1; i < F; ++i) HIPCHK(hipStreamWaitEvent(st[i], c->ev_fork, 0));
+        }
+        // longest bucket first, the passes dealt over the streams in turn;
+        // each block bucket's emit follows its pass on the same stream
+        int k = 0;
+        for (int b = pl.nb - 1; b >= 0; --b, ++k) {
             const Pass& p = pl.pass[b];
+            hipStream_t q = st[k % F];
             ewfa::AlignArgs a = base_args(c, words, pairs, results, max_len);
             fill_pass_args(a, p, c);
             a.list = bucketed ? lists + (size_t)b * n : nullptr;
             a.count = n;
             a.count_ptr = bucketed ? ctrl + CTRL_BCOUNT + b : nullptr;
             a.queue = ctrl + CTRL_BQUEUE + b;
-            a.defer_list = c->list_a.as<uint32_t>();
+            a.defer_list = sx.list_a;
             a.defer_count = ctrl + CTRL_DEFER_A;
             a.max_len = pl.blen[b];
             if (p.reg) {  // the register kernel writes its pairs' streams itself
-                a.runs = c->runs.as<uint8_t>();
+                a.runs = sx.runs + p.slot_off;
                 a.runs_slot = runs_slot_of(p);
                 a.cigars = cigars;
-                a.fallback_list = c->list_fb.as<uint32_t>();
+                a.fallback_list = sx.list_fb;
                 a.fallback_count = ctrl + CTRL_FALLBACK;
             }
-            CHK(enqueue_bound(c, a, p, ctrl + CTRL_BOUNDQ + b, n, s));
-            CHK(launch_pass(a, p, s));
+            CHK(enqueue_bound(c, a, p, ctrl + CTRL_BOUNDQ + b, n, q));
+            CHK(launch_pass(a, p, q));
+            if (!p.reg) {
+                ewfa::EmitArgs e{};
+                e.words = words;
+                e.pairs = pairs;
+                e.results = results;
+                e.ops = ops_base(c);
+                e.ops_stride = ewfa::ops_words(max_len + 1);
+                e.cigars = cigars;
+                e.list = bucketed ? lists + (size_t)b * n : nullptr;
+                e.count = n;
+                e.count_ptr = bucketed ? ctrl + CTRL_BCOUNT + b : nullptr;
+                CHK(emit(c, e, n, pl.blen[b], true, q));
+            }
+        }
+        for (int i = 1; i < F; ++i) {
+            HIPCHK(hipEventRecord(c->ev_join[i - 1], st[i]));
+            HIPCHK(hipStreamWaitEvent(s, c->ev_join[i - 1], 0));
         }
         if (timing) HIPCHK(hipEventRecord(c->ev_a1, s));
     }
     if (!(phase & PLAN_EMIT)) return EWFA_OK;
-    bool fused = false;
-    for (int b = pl.nb - 1; b >= 0; --b) {
-        if (pl.pass[b].reg) {
-            fused = true;
-            continue;
-        }
-        ewfa::EmitArgs e{};
-        e.words = words;
-        e.pairs = pairs;
-        e.results = results;
-        e.ops = ops_base(c);
-        e.ops_stride = ewfa::ops_words(max_len + 1);
-        e.cigars = cigars;
-        e.list = bucketed ? lists + (size_t)b * n : nullptr;
-        e.count = n;
-        e.count_ptr = bucketed ? ctrl + CTRL_BCOUNT + b : nullptr;
-        CHK(emit(c, e, n, pl.blen[b], true, s));
-    }
     if (fused) {
         // paths with a match run of 255 bases or more go through the emit kernel
         ewfa::EmitArgs e{};
@@ -624,7 +775,7 @@ int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_
         e.ops = ops_base(c);
         e.ops_stride = ewfa::ops_words(max_len + 1);
         e.cigars = cigars;
-        e.list = c->list_fb.as<uint32_t>();
+        e.list = sx.list_fb;
         e.count = n;
         e.count_ptr = ctrl + CTRL_FALLBACK;
         CHK(emit_list(c, e, n, max_len, s));
@@ -906,6 +1057,7 @@ int align_device_with_scratch(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa
 
 hipStream_t context_stream(ewfa_ctx_t* c) { return c ? c->stream : nullptr; }
 
+
 int enqueue_align(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t* d_pairs, size_t n, uint32_t max_len,
                   ewfa_result_t* d_results, int8_t* d_cigars, uint64_t* d_ops, size_t ops_bytes) {
     if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
@@ -1043,8 +1195,13 @@ int ewfa_create(ewfa_ctx_t** out, const ewfa_opts_t* opts) {
         rc = fail(EWFA_ERR_HIP, "hipStreamCreate failed");
     if (rc == EWFA_OK) {
         if (hipEventCreate(&c->ev_in) != hipSuccess || hipEventCreate(&c->ev_a0) != hipSuccess ||
-            hipEventCreate(&c->ev_a1) != hipSuccess || hipEventCreate(&c->ev_e1) != hipSuccess)
+            hipEventCreate(&c->ev_a1) != hipSuccess || hipEventCreate(&c->ev_e1) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess)
             rc = fail(EWFA_ERR_HIP, "hipEventCreate failed");
+        for (int i = 0; rc == EWFA_OK && i < ewfa_ctx::MAX_FANOUT - 1; ++i)
+            if (hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming) != hipSuccess)
+                rc = fail(EWFA_ERR_HIP, "aux stream or event creation failed");
     }
     if (rc == EWFA_OK) rc = c->ctrl.reserve(c, CTRL_WORDS * 4);
     if (rc == EWFA_OK && hipHostMalloc((void**)&c->host_ctrl, CTRL_WORDS * 4, hipHostMallocDefault) != hipSuccess)
@@ -1064,11 +1221,18 @@ int ewfa_destroy(ewfa_ctx_t* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->graph2) (void)hipGraphExecDestroy(c->graph2);
-    for (hipEvent_t e : {c->ev_in, c->ev_a0, c->ev_a1, c->ev_e1})
+    for (hipEvent_t e : {c->ev_in, c->ev_a0, c->ev_a1, c->ev_e1, c->ev_fork})
         if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < ewfa_ctx::MAX_FANOUT - 1; ++i) {
+        if (c->aux[i]) {
+            (void)hipStreamSynchronize(c->aux[i]);
+            (void)hipStreamDestroy(c->aux[i]);
+        }
+        if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     for (DevBuf* b : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->runs, &c->list_fb, &c->bound,
-                      &c->h_words, &c->h_pairs, &c->h_results, &c->h_cigars, &c->dense, &c->scan})
+                      &c->gslot, &c->h_words, &c->h_pairs, &c->h_results, &c->h_cigars, &c->dense, &c->scan})
         b->release();
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -1155,8 +1319,8 @@ int ewfa_context_memory(ewfa_ctx_t* c, size_t* bytes) {
     if (!c || !bytes) return fail(EWFA_ERR_ARG, "null argument");
     size_t b = 0;
     for (const DevBuf* d : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->runs,
-                            &c->list_fb, &c->bound, &c->h_words, &c->h_pairs, &c->h_results, &c->h_cigars,
-                            &c->dense, &c->scan})
+                            &c->list_fb, &c->bound, &c->gslot, &c->h_words, &c->h_pairs, &c->h_results,
+                            &c->h_cigars, &c->dense, &c->scan})
         b += d->bytes;
     *bytes = b;
     return EWFA_OK;
@@ -1164,16 +1328,21 @@ int ewfa_context_memory(ewfa_ctx_t* c, size_t* bytes) {
 
 int ewfa_max_distance(ewfa_ctx_t* c, uint32_t max_len) {
     if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
-    int most = max_dcap_for(seq_words_for(max_len), 0, lds_budget(c));
-    if (most < 0) return fail(EWFA_ERR_RANGE, "sequences of %u bases do not fit in LDS", max_len);
-    return std::min(most, (int)max_len + 1);
+    // passes beyond what LDS holds keep their slots in HBM, up to the budget
+    // where one backtrace slot reaches GSLOT_BT_MAX bytes
+    return (int)std::min<int64_t>((int64_t)max_len + 1, gslot_max_dcap());
 }
 
 int ewfa_set_flags(ewfa_ctx_t* c, int flags) {
     if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
-    if (flags & ~(EWFA_FLAG_TIMING | EWFA_FLAG_NO_GRAPH)) return fail(EWFA_ERR_ARG, "unknown flag");
+    if (flags & ~(EWFA_FLAG_TIMING | EWFA_FLAG_NO_GRAPH | EWFA_FLAG_SERIAL)) return fail(EWFA_ERR_ARG, "unknown flag");
     c->timing = (flags & EWFA_FLAG_TIMING) != 0;
     c->use_graph = (flags & EWFA_FLAG_NO_GRAPH) == 0;
+    const int fanout = (flags & EWFA_FLAG_SERIAL) ? 1 : ewfa_ctx::MAX_FANOUT;
+    if (fanout != c->fanout) {
+        c->fanout = fanout;
+        ++c->policy_gen;  // a captured plan of the other shape is captured again
+    }
     return EWFA_OK;
 }
 

@@ -85,6 +85,12 @@ struct AlignArgs {
     uint32_t* bound;
     // 16-bit offsets in LDS (the pass's 2 * (max_len + dcap) fits)
     int32_t off16;
+    // per-block slots in HBM (sequence copies and both offset generations,
+    // gslot_words each) for passes whose wavefront LDS cannot hold; null: the
+    // slot is in LDS.  The backtrace of such a pass is in HBM, stored through
+    // 64-bit addresses.
+    uint64_t* gslot;
+    uint64_t gslot_words;
 };
 
 constexpr uint32_t BOUND_NONE = 0xFFFFFFFFu;
@@ -202,9 +208,11 @@ hipError_t launch_bucketize(const ewfa_pair_t* pairs, uint32_t n, const BucketTa
 // `scan` holds 2 * ceil(n / COMPACT_BLOCK) words.
 hipError_t launch_compact(const ewfa_pair_t* pairs, const ewfa_result_t* results, const int8_t* cigars, size_t n,
                           uint64_t* scan, int8_t* dense, hipStream_t stream);
-// Occupancy of the align kernel for a configuration (blocks per CU).
+// Waves per pair of the passes with their slots in HBM.
+constexpr int GSLOT_WAVES = 16;
 // Resident blocks per CU of a block-kernel variant (0: backtrace in HBM, 1: in
-// LDS, 2: with distance bounds, 3: with 16-bit offsets).
+// LDS, 2: with distance bounds, 3: with 16-bit offsets, 4: slots in HBM, 5:
+// slots in HBM with distance bounds).
 int align_blocks_per_cu(int waves, size_t lds, int variant = 1);
 // Raises the dynamic LDS limit of every block-kernel variant to `lds` bytes
 // (never lowers it).

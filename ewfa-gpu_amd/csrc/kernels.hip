@@ -225,10 +225,17 @@ __device__ __forceinline__ uint32_t run_index(int st, int k, int band) {
 
 // ROUNDS: 2 for paths of up to 128 steps, 1 when the pass has at most 64
 // (distance at most 64).
-template <int ROUNDS>
-__device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, const ewfa_pair_t& pr, const uint8_t* R,
+// BAND (the banded pass, ROUNDS 1): the traced ops come in a register (the
+// 32-bit half word q of the op words, steps 16q..16q+15, in lane q of opsv)
+// rather than from the op scratch, and the stream is assembled in LDS (over
+// the first 256 bytes of the run rows, all of whose runs are read by then)
+// and written out as whole dwords, the terminator included.  The op scratch
+// is written only for a pair left to the emit kernel.
+template <int ROUNDS, bool BAND = false>
+__device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, const ewfa_pair_t& pr, uint8_t* R,
                                            int dist, int start, int status, int tk, int lane, int band,
-                                           bool longrun) {
+                                           bool longrun, int opsv = 0) {
+    static_assert(!BAND || ROUNDS == 1, "the LDS-staged stream holds paths of up to 64 steps");
     ewfa_result_t* res = a.results + pidx;
     if ((pr.cigar_off & 3u) || (pr.cigar_cap & 3u)) {
         if (lane == 0) *res = ewfa_result_t{dist, 0, EWFA_PAIR_BADARG, 0};
@@ -239,8 +246,14 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
     // op of step s (level s+1) in lane s%64 of round s/64; steps up to dist-1
     // lie in the words the traceback wrote
     const uint64_t* ow = a.ops + (size_t)pidx * a.ops_stride;
-    const int opr0 = lane < dist ? (int)((ow[lane >> 5] >> ((lane & 31) << 1)) & 3u) : 0;
-    const int opr1 = ROUNDS > 1 && 64 + lane < dist ? (int)((ow[(64 + lane) >> 5] >> ((lane & 31) << 1)) & 3u) : 0;
+    int opr0, opr1 = 0;
+    if constexpr (BAND) {
+        const uint32_t hw = (uint32_t)__builtin_amdgcn_ds_bpermute((lane >> 4) << 2, opsv);
+        opr0 = lane < dist ? (int)((hw >> ((lane & 15) << 1)) & 3u) : 0;
+    } else {
+        opr0 = lane < dist ? (int)((ow[lane >> 5] >> ((lane & 31) << 1)) & 3u) : 0;
+        opr1 = ROUNDS > 1 && 64 + lane < dist ? (int)((ow[(64 + lane) >> 5] >> ((lane & 31) << 1)) & 3u) : 0;
+    }
     const int nsteps = status == EWFA_PAIR_UNDEFINED ? 0 : dist;
     // step s's diagonal: k_s = tk + (sum of op - 2 over steps s..start-1);
     // steps 64..127 first, their total carried into steps 0..63.  The walk
@@ -281,6 +294,11 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
         }
     }
     if (__ballot(big)) {
+        if constexpr (BAND) {
+            // the emit kernel reads the ops from the scratch
+            uint32_t* ow32 = reinterpret_cast<uint32_t*>(a.ops + (size_t)pidx * a.ops_stride);
+            if (dist > 0 && lane <= 2 * ((dist - 1) >> 5) + 1) ow32[lane] = (uint32_t)opsv;
+        }
         if (lane == 0) {
             *res = ewfa_result_t{dist, 0, status, 0};
             a.fallback_list[atomicAdd(a.fallback_count, 1u)] = pidx;
@@ -289,6 +307,28 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
     }
     int8_t* out = a.cigars + pr.cigar_off;
     const uint32_t cap = pr.cigar_cap;
+    if constexpr (BAND) {
+        // at most 64 steps of at most 3 bytes (a run under 255 bases takes at
+        // most 2) and a final run: under 256 bytes
+        int8_t* S = reinterpret_cast<int8_t*>(R);
+        reinterpret_cast<uint32_t*>(S)[lane] = 0u;
+        group_sync<1>();
+        const bool act = 0 < nsteps && lane < nsteps && lane <= stop;
+        const int nb = act ? short_run_bytes(run0) + (opr0 != 0 ? 1 : 0) : 0;
+        const int incl = wave_incl_scan_i32(nb);
+        if (nb) put_run_op(S, 256u, (uint32_t)(incl - nb), run0, opr0);
+        const uint32_t body = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
+        const int fin_bytes = (status != EWFA_PAIR_UNDEFINED && stop == nsteps) ? short_run_bytes(fin) : 0;
+        const uint32_t len = body + (uint32_t)fin_bytes;
+        if (lane == 0 && fin_bytes) put_run_op(S, 256u, body, fin, 0);
+        group_sync<1>();
+        // whole dwords up to the terminator's, as the emit kernel writes them
+        const uint32_t wbytes = min((len & ~3u) + 4u, cap);
+        if ((uint32_t)lane * 4u < wbytes)
+            reinterpret_cast<uint32_t*>(out)[lane] = reinterpret_cast<const uint32_t*>(S)[lane];
+        if (lane == 0) *res = ewfa_result_t{dist, (int32_t)len, len > cap ? EWFA_PAIR_OVERFLOW : status, 0};
+        return;
+    }
     uint32_t base = 0;
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
@@ -315,7 +355,7 @@ __device__ __forceinline__ void fused_emit(const AlignArgs& a, uint32_t pidx, co
 template <bool FUSED>
 __device__ __forceinline__ void finish_pair(const AlignArgs& a, uint32_t pidx, const uint64_t* BT, int gtid,
                                             bool done, int d, int limit, int tk, const ewfa_pair_t* prp = nullptr,
-                                            const uint8_t* R = nullptr, int band = -1, bool longrun = false) {
+                                            uint8_t* R = nullptr, int band = -1, bool longrun = false) {
     ewfa_result_t* res = a.results + pidx;
     const int tka = tk < 0 ? -tk : tk;
     const int cap = a.max_distance;
@@ -469,7 +509,9 @@ __device__ __forceinline__ int extend_le3(const uint32_t* P, const uint32_t* T, 
 // 64c + 63 - d) over the indices ilo..ihi.  Without a bound (BOUNDED false)
 // the range is the whole level 0..2d, whose every chunk starts with a lane
 // inside it.
-template <int G, bool BT_LDS, bool BOUNDED, typename Off>
+// BT_PTR: the op ballots are stored through BT (LDS, or HBM with 64-bit
+// addresses), else through the buffer resource BTs with a 32-bit offset.
+template <int G, bool BT_PTR, bool BOUNDED, typename Off>
 __device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, int lane, const Off* cur,
                                             Off* nxt, const uint32_t* P32, const uint32_t* T32, int plen,
                                             int tlen, uint32_t base, uint64_t* BT, __amdgpu_buffer_rsrc_t BTs) {
@@ -524,7 +566,7 @@ __device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, i
                 first = lane == 0;
             }
             if (first) {
-                if constexpr (BT_LDS) {
+                if constexpr (BT_PTR) {
                     uint64_t* bq = BT + 2 * (size_t)(base + (uint32_t)c);
                     bq[0] = lo;
                     bq[1] = hi;
@@ -543,9 +585,12 @@ __device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, i
 // Off: the type of the LDS offsets, int32_t, or int16_t when twice the
 // largest offset a pass can reach (max_len + dcap) fits (half the LDS, more
 // pairs per CU).
-template <int G, bool BT_LDS, bool BOUNDED, typename Off = int32_t>
+// GSLOT: `slot` is the block's slot in HBM (sequence copies and offsets),
+// the backtrace in HBM stored through 64-bit addresses.
+template <int G, bool BT_LDS, bool BOUNDED, typename Off = int32_t, bool GSLOT = false>
 __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, uint64_t* bt_g,
                            int gtid) {
+    constexpr bool BT_PTR = BT_LDS || GSLOT;
     const int lane = gtid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(gtid >> 6);  // uniform in the wave
     const ewfa_pair_t pr = a.pairs[pidx];
@@ -566,8 +611,10 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
     Off* Y = X + (2 * dcap + 5);
     uint64_t* BT = BT_LDS ? slot + slot_bt_word(sw, dcap) : bt_g;
     // the HBM backtrace slot as a buffer: stores take their offset in a scalar
-    const __amdgpu_buffer_rsrc_t BTs =
-        __builtin_amdgcn_make_buffer_rsrc(bt_g, (short)0, (int)(a.bt_slot_words * 8u), BUF_DWORD3);
+    // (slots of passes with 32-bit byte offsets; the HBM-slot passes store
+    // through BT)
+    const __amdgpu_buffer_rsrc_t BTs = __builtin_amdgcn_make_buffer_rsrc(
+        bt_g, (short)0, BT_PTR ? 0 : (int)(a.bt_slot_words * 8u), BUF_DWORD3);
 
     // stage the two sequences little-endian (coalesced 8-byte loads) plus
     // two zero pad words
@@ -614,9 +661,9 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
             // cells within the bound
             const int ilo = max(0, tk + d - (U - d));
             const int ihi = min(2 * d, tk + d + (U - d));
-            block_level<G, BT_LDS, true, Off>(d, ilo, ihi, wave, lane, cur, nxt, P32, T32, plen, tlen, base, BT, BTs);
+            block_level<G, BT_PTR, true, Off>(d, ilo, ihi, wave, lane, cur, nxt, P32, T32, plen, tlen, base, BT, BTs);
         } else {
-            block_level<G, BT_LDS, false, Off>(d, 0, 2 * d, wave, lane, cur, nxt, P32, T32, plen, tlen, base, BT, BTs);
+            block_level<G, BT_PTR, false, Off>(d, 0, 2 * d, wave, lane, cur, nxt, P32, T32, plen, tlen, base, BT, BTs);
         }
         // sentinels for the next level's reads at |k| = d+1, d+2
         if (gtid < 4) {
@@ -900,7 +947,7 @@ __device__ __forceinline__ int ballot_op32(uint32_t L, uint32_t S, int bit) {
 // sl/sh (SUB at least INS; an op is DEL if its DEL bit is set, else SUB if
 // its SUB bit is, else INS).  Status, distance and deferral as finish_pair.
 __device__ __forceinline__ void finish_pair_band(const AlignArgs& a, uint32_t pidx, int lane, bool done, int d,
-                                                 int limit, int tk, const ewfa_pair_t& pr, const uint8_t* R,
+                                                 int limit, int tk, const ewfa_pair_t& pr, uint8_t* R,
                                                  int band, bool longrun, int dl, int dh, int sl, int sh) {
     ewfa_result_t* res = a.results + pidx;
     const int tka = tk < 0 ? -tk : tk;
@@ -923,42 +970,36 @@ __device__ __forceinline__ void finish_pair_band(const AlignArgs& a, uint32_t pi
             return;
         }
     }
-    uint32_t* ow32 = reinterpret_cast<uint32_t*>(a.ops + (size_t)pidx * a.ops_stride);
-    if (dist > 0) {
+    // the traced ops: 32-bit half word q (steps 16q..16q+15, step s at bits
+    // 2(s%16)) in lane q; half words above the walked ones hold zero ops
+    int opsv = 0;
+    if (dist > 0 && tka <= start && start >= 1) {
         // steps start-1 down to 0 (op s from level s+1) when the target
-        // diagonal is inside the last level, else none; each 32-bit half of
-        // an ops word (16 steps, step s at bits 2(s%16)) fills from its top
-        // step down, shifting in two bits a step
-        int walked_top = -1;
-        if (tka <= start && start >= 1) {
-            walked_top = (start - 1) >> 4;
-            // the path's diagonal, as a ballot bit (kept in a scalar register)
-            int bit = __builtin_amdgcn_readfirstlane(tk + band);
-            for (int s = start - 1; s >= 0;) {
-                const int lo = s & ~15;
-                uint32_t hw = 0;
-                for (; s >= lo; --s) {
-                    // the ballot half that holds the path's bit
-                    uint32_t L, S;
-                    if (bit >= 32) {
-                        L = (uint32_t)__builtin_amdgcn_readlane(dh, s);
-                        S = (uint32_t)__builtin_amdgcn_readlane(sh, s);
-                    } else {
-                        L = (uint32_t)__builtin_amdgcn_readlane(dl, s);
-                        S = (uint32_t)__builtin_amdgcn_readlane(sl, s);
-                    }
-                    const int op = ballot_op32(L, S, bit);
-                    hw = (hw << 2) + (uint32_t)op;
-                    bit += op - 2;
+        // diagonal is inside the last level, else none; each half word fills
+        // from its top step down, shifting in two bits a step
+        // the path's diagonal, as a ballot bit (kept in a scalar register)
+        int bit = __builtin_amdgcn_readfirstlane(tk + band);
+        for (int s = start - 1; s >= 0;) {
+            const int lo = s & ~15;
+            uint32_t hw = 0;
+            for (; s >= lo; --s) {
+                // the ballot half that holds the path's bit
+                uint32_t L, S;
+                if (bit >= 32) {
+                    L = (uint32_t)__builtin_amdgcn_readlane(dh, s);
+                    S = (uint32_t)__builtin_amdgcn_readlane(sh, s);
+                } else {
+                    L = (uint32_t)__builtin_amdgcn_readlane(dl, s);
+                    S = (uint32_t)__builtin_amdgcn_readlane(sl, s);
                 }
-                if (lane == 0) ow32[lo >> 4] = hw;
+                const int op = ballot_op32(L, S, bit);
+                hw = (hw << 2) + (uint32_t)op;
+                bit += op - 2;
             }
+            opsv = writelane_i32(opsv, (int)hw, lo >> 4);
         }
-        // the half words above the walked ones, up to step dist-1, hold zero ops
-        for (int q = 2 * ((dist - 1) >> 5) + 1; q > walked_top; --q)
-            if (lane == 0) ow32[q] = 0;
     }
-    fused_emit<1>(a, pidx, pr, R, dist, start, status, tk, lane, band, longrun);  // dist <= 64
+    fused_emit<1, true>(a, pidx, pr, R, dist, start, status, tk, lane, band, longrun, opsv);  // dist <= 64
 }
 
 // Banded register wavefront: one VGPR of 64 diagonals for passes that search
@@ -1198,18 +1239,20 @@ __global__ __launch_bounds__(64, NC <= 3 ? 8 : 6) void align_reg_kernel(AlignArg
 // ---------------------------------------------------------------------------
 // align kernel: G waves per pair, one pair slot per block
 
-template <int G, bool BT_LDS, bool BOUNDED = false, bool OFF16 = false>
+// GSLOT: each block's slot (sequence copies, offsets) is its gslot_words of
+// a.gslot in HBM instead of LDS: the passes whose wavefront LDS cannot hold.
+template <int G, bool BT_LDS, bool BOUNDED = false, bool OFF16 = false, bool GSLOT = false>
 __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persistent) {
     extern __shared__ uint64_t lds[];
     __shared__ uint32_t s_base;
     const int tid = threadIdx.x;
     uint64_t* btg = BT_LDS ? nullptr : a.bt_global + (size_t)blockIdx.x * a.bt_slot_words;
+    uint64_t* slot = GSLOT ? a.gslot + (size_t)blockIdx.x * a.gslot_words : lds;
     const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
+    using Off = typename std::conditional<OFF16, int16_t, int32_t>::type;
     if (!persistent) {
         const uint32_t item = blockIdx.x;
-        if (item < count)
-            align_pair<G, BT_LDS, BOUNDED, typename std::conditional<OFF16, int16_t, int32_t>::type>(
-                a, a.list ? a.list[item] : item, lds, btg, tid);
+        if (item < count) align_pair<G, BT_LDS, BOUNDED, Off, GSLOT>(a, a.list ? a.list[item] : item, slot, btg, tid);
         return;
     }
     for (;;) {
@@ -1227,8 +1270,7 @@ __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persiste
         if (base >= count) break;
         const uint32_t end = min(base + a.grab, count);
         for (uint32_t it = base; it < end; ++it) {
-            align_pair<G, BT_LDS, BOUNDED, typename std::conditional<OFF16, int16_t, int32_t>::type>(
-                a, a.list ? a.list[it] : it, lds, btg, tid);
+            align_pair<G, BT_LDS, BOUNDED, Off, GSLOT>(a, a.list ? a.list[it] : it, slot, btg, tid);
             group_sync<G>();  // the slot is reused by the next pair
         }
     }
@@ -1859,6 +1901,17 @@ template <int G, bool BT_LDS>
 static hipError_t launch_align_t(const AlignArgs& a, bool persistent, uint32_t grid, size_t lds,
                                  hipStream_t stream) {
     if (a.bound && a.off16) return hipErrorInvalidValue;  // the bounded kernel keeps 32-bit offsets
+    if (a.gslot) {
+        // slots in HBM: GSLOT_WAVES waves, 32-bit offsets, no dynamic LDS
+        if (G != GSLOT_WAVES || BT_LDS || a.off16 || lds) return hipErrorInvalidValue;
+        if (a.bound)
+            hipLaunchKernelGGL((align_kernel<GSLOT_WAVES, false, true, false, true>), dim3(grid),
+                               dim3(64 * GSLOT_WAVES), 0, stream, a, persistent ? 1 : 0);
+        else
+            hipLaunchKernelGGL((align_kernel<GSLOT_WAVES, false, false, false, true>), dim3(grid),
+                               dim3(64 * GSLOT_WAVES), 0, stream, a, persistent ? 1 : 0);
+        return hipGetLastError();
+    }
     if (a.bound && !BT_LDS)
         hipLaunchKernelGGL((align_kernel<G, false, true>), dim3(grid), dim3(64 * G), lds, stream, a,
                            persistent ? 1 : 0);
@@ -1931,6 +1984,12 @@ static const void* align_fn_variant(int variant) {
         case 1: return reinterpret_cast<const void*>(&align_kernel<G, true>);
         case 2: return reinterpret_cast<const void*>(&align_kernel<G, false, true>);
         case 3: return reinterpret_cast<const void*>(&align_kernel<G, false, false, true>);
+        case 4:
+            if constexpr (G == GSLOT_WAVES) return reinterpret_cast<const void*>(&align_kernel<G, false, false, false, true>);
+            return nullptr;
+        case 5:
+            if constexpr (G == GSLOT_WAVES) return reinterpret_cast<const void*>(&align_kernel<G, false, true, false, true>);
+            return nullptr;
         default: return nullptr;
     }
 }

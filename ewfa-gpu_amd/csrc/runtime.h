@@ -14,6 +14,22 @@ namespace ewfa {
 // returns `code`.
 int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 
+// Device scratch of one persistent align call.  bt and runs share one region
+// of `region` bytes (the passes of a call run one after another).
+struct Scratch {
+    uint64_t* ops;       // op words, ops_words(max_len + 1) per pair
+    uint32_t* ctrl;      // control words (queue heads, counts)
+    uint32_t* list_a;    // deferral lists (ping-pong) and the emit fallback list
+    uint32_t* list_b;
+    uint32_t* list_fb;
+    uint32_t* buckets;   // per-bucket pair lists
+    uint32_t* bound;     // per-pair distance bounds
+    uint64_t* bt;        // HBM backtrace slots of the block passes
+    uint8_t* runs;       // run rows of the multi-chunk register passes
+    uint64_t* gslot;     // HBM slots of the passes beyond what LDS holds
+    size_t region;
+};
+
 // Op scratch bytes an align call over n pairs up to max_len bases needs.
 size_t op_scratch_bytes(size_t n, uint32_t max_len);
 

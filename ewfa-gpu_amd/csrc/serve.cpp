@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -300,24 +301,58 @@ int enqueue_batch(Slot* sl, Flight* f) {
     return EWFA_OK;
 }
 
+// Runs fn(k) for k in [0, count) on up to `threads` threads (the calling one
+// included), in contiguous ranges.
+template <typename F>
+void parallel_ranges(size_t count, int threads, F&& fn) {
+    const size_t t = std::max<size_t>(1, std::min<size_t>((size_t)threads, count));
+    if (t == 1) {
+        for (size_t k = 0; k < count; ++k) fn(k);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const size_t per = (count + t - 1) / t;
+    for (size_t i = 1; i < t; ++i)
+        pool.emplace_back([&fn, i, per, count] {
+            for (size_t k = i * per; k < std::min(count, (i + 1) * per); ++k) fn(k);
+        });
+    for (size_t k = 0; k < std::min(count, per); ++k) fn(k);
+    for (std::thread& th : pool) th.join();
+}
+
+// Threads that hand out one batch: one per 4,096 pairs, at most 8.
+int handout_threads(size_t pairs) { return (int)std::min<size_t>(8, std::max<size_t>(1, pairs / 4096)); }
+
 // After the batch's device work (results and compacted CIGAR bytes are in the
 // flight's pinned buffers): each job gets its results and its streams placed
-// at its own offsets.
-int deliver_batch(Flight* f) {
+// at its own offsets, jobs in parallel, and completes (`done(job, rc)`) as
+// soon as its own outputs are in place.
+template <typename Done>
+int deliver_batch(Flight* f, Done&& done) {
     const Batch& b = f->b;
-    if (b.n == 0) return EWFA_OK;
-    if (ewfa::compact_total(f->pairs.p, f->results.p, b.n) > b.caps) return EWFA_ERR_STATE;
-    const int8_t* dense = f->dense.p;
-    size_t left = b.caps, k = 0;
-    for (Job* j : b.jobs) {
-        memcpy(j->results, f->results.p + k, j->n * sizeof(ewfa_result_t));
-        const size_t w = ewfa::scatter_streams(dense, left, j->pairs, j->results, j->n, j->cigars, 1);
-        if (w == SIZE_MAX) return EWFA_ERR_STATE;
-        dense += w;
-        left -= w;
-        k += j->n;
+    const size_t nj = b.jobs.size();
+    // each job's first pair and first dense byte
+    std::vector<size_t> first(nj + 1, 0), dense_at(nj + 1, 0);
+    for (size_t k = 0; k < nj; ++k) first[k + 1] = first[k] + b.jobs[k]->n;
+    parallel_ranges(nj, handout_threads(b.n), [&](size_t k) {
+        dense_at[k + 1] = ewfa::compact_total(f->pairs.p + first[k], f->results.p + first[k], b.jobs[k]->n);
+    });
+    for (size_t k = 0; k < nj; ++k) dense_at[k + 1] += dense_at[k];
+    if (dense_at[nj] > b.caps) {
+        for (Job* j : b.jobs) done(j, EWFA_ERR_STATE);
+        return EWFA_ERR_STATE;
     }
-    return EWFA_OK;
+    std::atomic<int> bad{0};
+    parallel_ranges(nj, handout_threads(b.n), [&](size_t k) {
+        Job* j = b.jobs[k];
+        memcpy(j->results, f->results.p + first[k], j->n * sizeof(ewfa_result_t));
+        const size_t w = ewfa::scatter_streams(f->dense.p + dense_at[k], dense_at[k + 1] - dense_at[k], j->pairs,
+                                               j->results, j->n, j->cigars, 1);
+        const int rc = w == SIZE_MAX ? EWFA_ERR_STATE : EWFA_OK;
+        if (rc != EWFA_OK) bad = 1;
+        done(j, rc);
+    });
+    return bad ? EWFA_ERR_STATE : EWFA_OK;
 }
 
 void worker_main(Slot* sl) {
@@ -391,21 +426,29 @@ void completer_main(Slot* sl) {
             (void)hipStreamSynchronize(sl->cs);
         }
         const double al = since(t);
-        if (rc == EWFA_OK) rc = deliver_batch(f);
+        // a job completes as soon as its own results and streams are placed;
+        // its resident block is freed at once
+        const auto done = [&](Job* j, int jrc) {
+            std::lock_guard<std::mutex> lk(s->m);
+            s->arena.free(j->data_off, j->data_bytes);
+            j->rc = jrc;
+            j->finished = Clock::now();
+            j->done = true;
+            ++s->completed;
+            if (jrc != EWFA_OK) ++s->failed;
+            --s->running;
+            s->cv_done.notify_all();
+            s->cv_work.notify_all();  // freed arena space may admit the queue's head
+        };
+        if (rc == EWFA_OK) {
+            (void)deliver_batch(f, done);
+        } else {
+            for (Job* j : f->b.jobs) done(j, rc);
+        }
         const double ho = since(t);
         {
             std::lock_guard<std::mutex> lk(s->m);
-            const auto now = Clock::now();
-            for (Job* j : f->b.jobs) {
-                s->arena.free(j->data_off, j->data_bytes);
-                j->rc = rc;
-                j->finished = now;
-                j->done = true;
-                ++s->completed;
-                if (rc != EWFA_OK) ++s->failed;
-            }
             s->arena.free(f->b.state_off, f->b.state);
-            s->running -= (int)f->b.jobs.size();
             s->align_s += al;
             s->handout_s += ho;
         }
@@ -429,7 +472,9 @@ int slot_init(ewfa_server* s, Slot* sl) {
     o.policy_path = s->policy.empty() ? nullptr : s->policy.c_str();
     int rc = ewfa_create(&sl->ctx, &o);
     if (rc != EWFA_OK) return rc;
-    ewfa_set_flags(sl->ctx, EWFA_FLAG_NO_GRAPH);
+    // eager launches (each batch differs); a batch's bucket passes stay on
+    // the slot's stream, the slots themselves running side by side
+    ewfa_set_flags(sl->ctx, EWFA_FLAG_NO_GRAPH | EWFA_FLAG_SERIAL);
     sl->cs = ewfa::context_stream(sl->ctx);
     if (hipSetDevice(s->opts.device) != hipSuccess ||
         hipStreamCreateWithFlags(&sl->up, hipStreamNonBlocking) != hipSuccess)
@@ -462,7 +507,7 @@ int ewfa_server_create(ewfa_server_t** out, const ewfa_server_opts_t* opts) {
     if (ewfa_device_count() <= 0) return EWFA_ERR_NODEV;
     ewfa_server* s = new ewfa_server();
     if (opts) s->opts = *opts;
-    if (s->opts.slots <= 0) s->opts.slots = 1;
+    if (s->opts.slots <= 0) s->opts.slots = 2;
     if (s->opts.slots > 64) s->opts.slots = 64;
     if (s->opts.arena_bytes == 0) s->opts.arena_bytes = (size_t)4 << 30;
     if (s->opts.max_batch_pairs == 0) s->opts.max_batch_pairs = (size_t)1 << 17;

@@ -21,7 +21,7 @@ OK = 0
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NODEV, ERR_IO, ERR_FORMAT, ERR_RANGE, ERR_STATE = range(-1, -9, -1)
 PAIR_OK, PAIR_CAPPED, PAIR_OVERFLOW, PAIR_UNDEFINED, PAIR_BADARG, PAIR_TOO_LONG, PAIR_TOO_DISTANT = range(7)
 MODE_PLAIN, MODE_PERSISTENT = 0, 1
-FLAG_TIMING, FLAG_NO_GRAPH = 1, 2
+FLAG_TIMING, FLAG_NO_GRAPH, FLAG_SERIAL = 1, 2, 4
 
 PAIR_DTYPE = np.dtype([
     ("pattern_off", "<u8"), ("text_off", "<u8"), ("cigar_off", "<u8"),

@@ -50,8 +50,7 @@ extern "C" {
                                    stream is left empty */
 #define EWFA_PAIR_BADARG     4  /* descriptor invalid (alignment, capacity) */
 #define EWFA_PAIR_TOO_LONG   5  /* plen/tlen above the call's max_len */
-#define EWFA_PAIR_TOO_DISTANT 6 /* distance beyond the largest wavefront the
-                                   device can stage (see ewfa_max_distance) */
+#define EWFA_PAIR_TOO_DISTANT 6 /* distance beyond ewfa_max_distance */
 
 /* One pair: the replacement of WF_element (src/wavefront_structures.h:44-50)
  * plus the addresses PATTERN_PTR/TEXT_PTR and Cigars::get_cigar_ascii derive
@@ -175,13 +174,20 @@ int ewfa_last_timing(ewfa_ctx_t *ctx, float *align_ms, float *emit_ms);
                                   emit kernels (inside the captured graph on
                                   the persistent path) */
 #define EWFA_FLAG_NO_GRAPH  2  /* launch the persistent path eagerly */
+#define EWFA_FLAG_SERIAL    4  /* run the persistent plan's length-bucket
+                                  passes one after another on the context's
+                                  stream (default: spread over three streams,
+                                  so one pass's tail runs beside the next) */
 int ewfa_set_flags(ewfa_ctx_t *ctx, int flags);
 
 /* Device bytes the context holds now: op scratch, backtrace spill buffer,
  * bucket and deferral lists, control words and the host-pipeline buffers. */
 int ewfa_context_memory(ewfa_ctx_t *ctx, size_t *bytes);
 
-/* Largest distance the device path can resolve for sequences up to max_len. */
+/* Largest distance the device path resolves for sequences up to max_len:
+ * max_len + 1 (every distance) up to about 131,000.  Wavefronts beyond what
+ * LDS holds run in deferral passes whose slots live in HBM; their backtrace
+ * takes (D+1)^2/4 bytes per pair in flight, which sets that limit. */
 int ewfa_max_distance(ewfa_ctx_t *ctx, uint32_t max_len);
 
 /* ---- serving -------------------------------------------------------------- */
@@ -193,8 +199,10 @@ int ewfa_max_distance(ewfa_ctx_t *ctx, uint32_t max_len);
  * together (and at most max_batch_pairs pairs and max_batch_cigar_bytes of
  * CIGAR regions, a single larger job excepted), and enqueues them as ONE
  * persistent launch without waiting; jobs that arrive meanwhile form the next
- * batch, queued behind it on the device.  Only the written CIGAR bytes cross
- * PCIe, straight into pinned host memory.  A job's footprint in the arena is
+ * batch, on another slot's stream (where it runs beside the first on the
+ * device) or queued behind it.  Only the written CIGAR bytes cross PCIe,
+ * straight into pinned host memory, and each job completes as soon as its own
+ * results and streams are placed in its buffers.  A job's footprint in the arena is
  * its packed sequences and CIGAR regions (resident while it runs) plus its
  * share of the batch's launch state: descriptors, results and the traceback
  * op scratch for its pairs at the batch's longest sequence.  Every job's
@@ -203,8 +211,10 @@ typedef struct ewfa_server ewfa_server_t;
 
 typedef struct ewfa_server_opts {
     int device;
-    int slots;               /* device contexts, each with two batches in
-                                flight (0: 1) */
+    int slots;               /* device contexts, each with its own compute
+                                stream and two batches in flight; batches of
+                                different slots run side by side on the
+                                device (0: 2) */
     size_t arena_bytes;      /* device arena for job data and launch state (0: 4 GiB) */
     int max_distance;        /* as ewfa_opts_t */
     const char *policy_path; /* as ewfa_opts_t */

@@ -243,26 +243,39 @@ def test_cigar_overflow_status():
             assert cig[8 * i: 8 * i + full].tolist() == st[i].tolist()
 
 
-def test_too_distant_beyond_lds_budget():
-    """A pair whose distance is beyond every wavefront budget LDS can hold for
-    its length ends EWFA_PAIR_TOO_DISTANT; a close pair in the same call is
-    aligned as usual."""
+def test_distances_beyond_what_lds_holds():
+    """Pairs whose wavefront no longer fits LDS go to deferral passes whose
+    slots (sequence copies and offsets) live in HBM: a 100 kbp pair at 10%
+    edits (distance about 9,600, above the about 7,100 levels LDS holds for
+    that length) equals the oracle bit for bit, and a random 40 kbp pair
+    (distance about 21,000) has the Levenshtein distance and a valid stream
+    with as many edits; a close pair in the same call is aligned as usual."""
     rng = np.random.default_rng(21)
     L = 40000
     a = bytes(rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), L))
     b = bytes(rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), L))
     c = a[:2000]
     d = c[:1000] + b"A" + c[1000:]
-    say("case too distant: random 40 kbp pair and a one-insertion pair")
-    words, pairs, cb, ml = ewfa.from_sequences([a, c], [b, d])
+    w = ewfa.Workload(ewfa.synth_spec(kind=0, seed=77, len=100_000, err=0.10), 1)
+    p100, t100 = w.pattern(0), w.text(0)
+    say("case beyond LDS: 100 kbp pair at 10%, random 40 kbp pair, a one-insertion pair")
+    words, pairs, cb, ml = ewfa.from_sequences([p100, a, c], [t100, b, d])
     with ewfa.Context() as ctx:
-        most = ctx.max_distance(ml)
-        res, cig = ctx.align_host(words, pairs, 2, cb)
-    assert most < L // 3
-    assert int(res[0]["status"]) == ewfa.PAIR_TOO_DISTANT
-    assert int(res[1]["status"]) == ewfa.PAIR_OK and int(res[1]["distance"]) == 1
-    dist, stream = orc.stream(c, d)
-    assert ewfa.streams_from(res[1:], pairs[1:], cig)[0].tolist() == stream
+        assert ctx.max_distance(ml) == ml + 1
+        res, cig = ctx.align_host(words, pairs, 3, cb)
+    streams = ewfa.streams_from(res, pairs, cig)
+    assert all(int(r["status"]) == ewfa.PAIR_OK for r in res)
+    say(f"  distances {res['distance'].tolist()}; oracle and DP on the host")
+    # the 100 kbp pair: the oracle's distance and stream
+    ok, od, osl, ost = orc.batch_streams([p100], [t100], threads=1)
+    assert int(res[0]["distance"]) == int(od[0]) > 8000
+    assert np.array_equal(streams[0], ost[0])
+    # the random pair: exact distance, a valid alignment with that many edits
+    assert int(res[1]["distance"]) == orc.levenshtein(a, b)
+    assert ewfa.check_cigar(a, b, streams[1]) == 1
+    assert int(np.isin(streams[1], [ord("X"), ord("I"), ord("D")]).sum()) == int(res[1]["distance"])
+    assert int(res[2]["distance"]) == 1
+    assert streams[2].tolist() == orc.stream(c, d)[1]
 
 
 def test_host_pipeline_several_chunks():
