@@ -367,14 +367,22 @@ def leg_cli(args, w, threads):
     ("Aligned executed. Wall time", src/aligner.cu:233) with and without -p,
     a sample of the printed lines against the oracle's text, and the pair-file
     ingestion (file mapped, lines found, sequences packed) timed apart."""
+    import shutil
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="ewfa_cli_")
+    try:
+        return cli_runs(args, w, threads, tmp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def cli_runs(args, w, threads, tmp):
     import ctypes
     import re
     import subprocess
-    import tempfile
     import numpy as np
     import ewfa
     import oracle_py as orc
-    tmp = tempfile.mkdtemp(prefix="ewfa_cli_")
     path = os.path.join(tmp, "c2_pairs.txt")
     t0 = time.time()
     size = write_pair_file(w, path)
@@ -419,8 +427,6 @@ def leg_cli(args, w, threads):
             out[key]["lines_checked"] = len(idx)
             out[key]["mismatches"] = bad + len(want)
         os.unlink(sink)
-    os.unlink(path)
-    os.rmdir(tmp)
     return out
 
 
@@ -572,7 +578,21 @@ def main():
 
     legs = set(args.legs.split(","))
     c2 = leg_c2(args, threads)
-    cli = leg_cli(args, c2["w"], threads) if "cli" in legs else None
+    cli = None
+    if "cli" in legs:
+        # the pair file takes about 2 GB of temporary space; a box without
+        # it (or any failure of this leg) is reported, not fatal
+        import shutil
+        import tempfile
+        free = shutil.disk_usage(tempfile.gettempdir()).free
+        if free < 6 << 30:
+            cli = {"skipped": f"{free / 2**30:.1f} GiB free in {tempfile.gettempdir()}, 6 needed"}
+        else:
+            try:
+                cli = leg_cli(args, c2["w"], threads)
+            except Exception as e:  # reported in the JSON line
+                cli = {"error": f"{type(e).__name__}: {e}"[:300]}
+        log(f"cli: {cli if 'skipped' in cli or 'error' in cli else 'done'}")
     c3 = leg_c3(args, threads) if "c3" in legs else None
     c4 = leg_c4(args, threads) if "c4" in legs else None
     served = leg_served(args, threads, c3["value"]) if ("served" in legs and c3) else None

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <condition_variable>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -36,6 +37,7 @@ std::mutex g_out_mutex;
 
 struct Params {
     const ewfa_pairs_file_t* file;
+    const uint64_t* words;  // the file's packed words, in pinned memory
     int64_t seq_len;
     size_t num_alignments;
     size_t batch_size;
@@ -62,11 +64,30 @@ uint32_t exact_capacity(const ewfa_pair_t& p) {
 }
 
 // One batch worth of host buffers.
+// A host buffer that only grows and is never zero-filled: the streams are
+// read only up to the bytes the library wrote.
+struct Bytes {
+    std::unique_ptr<int8_t[]> p;
+    size_t n = 0, cap = 0;
+    int8_t* data() { return p.get(); }
+    const int8_t* data() const { return p.get(); }
+    size_t size() const { return n; }
+    void resize(size_t m) {
+        if (m > cap) {
+            std::unique_ptr<int8_t[]> q(new int8_t[m]);
+            if (n) memcpy(q.get(), p.get(), n);
+            p = std::move(q);
+            cap = m;
+        }
+        n = m;
+    }
+};
+
 struct Batch {
     size_t first = 0;   // global index of the batch's first pair
     size_t count = 0;
     std::vector<ewfa_result_t> results;
-    std::vector<int8_t> cigars;
+    Bytes cigars;
     std::vector<ewfa_pair_t> pairs;
 };
 
@@ -186,6 +207,20 @@ void* worker(void* arg) {
     }
     const ewfa_pairs_file_t& f = *p.file;
     const uint32_t cap = cigar_capacity(p.seq_len);
+    // this worker's pairs read one contiguous range of the packed words (the
+    // reader lays them out pair by pair): it goes up once, and every batch
+    // aligns against it (the reference uploads each batch's sequences,
+    // src/wavefront.cu:265-284)
+    const uint64_t w0 = f.pairs[skip].pattern_off;
+    const ewfa_pair_t& last = f.pairs[skip + todo - 1];
+    const uint64_t w1 = last.text_off + ewfa_packed_words(last.tlen);
+    if (ewfa_upload_words(ctx, p.words + w0, (size_t)(w1 - w0)) != EWFA_OK) {
+        std::lock_guard<std::mutex> lk(g_out_mutex);
+        fprintf(stderr, "%s\n", ewfa_last_error());
+        p.rc = 1;
+        ewfa_destroy(ctx);
+        return nullptr;
+    }
 
     // a printer thread formats batch b while the device aligns batch b+1
     std::mutex m;
@@ -218,15 +253,15 @@ void* worker(void* arg) {
         b.count = std::min(batch, todo - off);
         b.pairs.assign(f.pairs + b.first, f.pairs + b.first + b.count);
         for (size_t i = 0; i < b.count; ++i) {
+            b.pairs[i].pattern_off -= w0;
+            b.pairs[i].text_off -= w0;
             b.pairs[i].cigar_off = (uint64_t)i * cap;
             b.pairs[i].cigar_cap = cap;
         }
         b.results.resize(b.count);
         b.cigars.resize((size_t)b.count * cap);
-        // each call uploads the words its pairs read, chunk by chunk, while
-        // the chunk before computes
-        int rc = ewfa_align_host(ctx, f.words, f.n_words, b.pairs.data(), b.count, b.results.data(),
-                                 b.cigars.data(), b.cigars.size());
+        int rc = ewfa_align_resident(ctx, b.pairs.data(), b.count, b.results.data(), b.cigars.data(),
+                                     b.cigars.size());
         if (rc == EWFA_OK) {
             // streams longer than the reference's buffer: align again with
             // room for any stream of the pair
@@ -244,8 +279,7 @@ void* worker(void* arg) {
                     total += op[j].cigar_cap;
                 }
                 std::vector<int8_t> oc(total);
-                rc = ewfa_align_host(ctx, f.words, f.n_words, op.data(), op.size(), orr.data(), oc.data(),
-                                     oc.size());
+                rc = ewfa_align_resident(ctx, op.data(), op.size(), orr.data(), oc.data(), oc.size());
                 if (rc == EWFA_OK) {
                     // append the long streams after the batch's buffer
                     const size_t base = b.cigars.size();
@@ -331,6 +365,23 @@ int main(int argc, char** argv) {
         exit(1);
     }
 
+    // the packed words in pinned memory (the reference reads into pinned
+    // memory too, utils/sequence_reader.cu:80), copied before the timer starts
+    uint64_t* pinned = nullptr;
+    if (ewfa_host_alloc(std::max<size_t>(f.n_words, 1) * 8, (void**)&pinned) != EWFA_OK)
+        fatal("Could not allocate memory.\n");
+    {
+        const size_t nt = std::min<size_t>(std::min(hw, 16u), std::max<size_t>(1, f.n_words >> 20));
+        std::vector<std::thread> pool;
+        const size_t per = (f.n_words + nt - 1) / nt;
+        for (size_t t = 0; t < nt; ++t)
+            pool.emplace_back([&, t] {
+                const size_t lo = std::min(f.n_words, t * per), hi = std::min(f.n_words, (t + 1) * per);
+                if (hi > lo) memcpy(pinned + lo, f.words + lo, (hi - lo) * 8);
+            });
+        for (std::thread& th : pool) th.join();
+    }
+
     // formatting threads per worker: the host's cores shared by the -t workers
     g_format_threads = (int)std::max(1u, std::min(hw, 16u) / std::max(1u, num_threads));
 
@@ -341,6 +392,7 @@ int main(int argc, char** argv) {
     for (unsigned i = 0; i < num_threads; ++i) {
         Params& p = params[i];
         p.file = &f;
+        p.words = pinned;
         p.seq_len = a.seq_len;
         p.num_alignments = num_alignments;
         p.batch_size = batch_size;
@@ -364,6 +416,7 @@ int main(int argc, char** argv) {
     printf("Aligned executed. Wall time %fs\n", seconds);
     int status = 0;
     for (const Params& p : params) status |= p.rc;
+    ewfa_host_free(pinned);
     ewfa_free_pairs(&f);
     return status;
 }
