@@ -1057,6 +1057,86 @@ int align_device_with_scratch(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa
 
 hipStream_t context_stream(ewfa_ctx_t* c) { return c ? c->stream : nullptr; }
 
+int context_device(ewfa_ctx_t* c) { return c ? c->device : -1; }
+
+int align_device_in(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t* d_pairs, size_t n, uint32_t max_len,
+                    ewfa_result_t* d_results, int8_t* d_cigars, void* stream, void* scratch, size_t bytes) {
+    if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    if (c->opts.mode != EWFA_MODE_PERSISTENT) return fail(EWFA_ERR_STATE, "the allocation-free call needs the persistent mode");
+    if (n == 0) return EWFA_OK;
+    if (!d_words || !d_pairs || !d_results || !d_cigars || !scratch) return fail(EWFA_ERR_ARG, "null buffer");
+    if (n > 0xFFFFFFF0u) return fail(EWFA_ERR_RANGE, "too many pairs in one call");
+    CHK(set_device(c));
+    Plan pl;
+    CHK(plan_persistent(c, max_len, (uint32_t)n, pl));
+    // one stream: every pass's slots share the region.  The static scratch
+    // has no room for slots in HBM: the deferral chain stops before the first
+    // such pass (its pairs come back TOO_DISTANT), and sequences too long for
+    // LDS are refused.
+    for (int b = 0; b < pl.nb; ++b)
+        if (pl.pass[b].gslot)
+            return fail(EWFA_ERR_RANGE, "sequences of %u bases do not fit in LDS, which the allocation-free call "
+                        "needs", max_len);
+    for (int k = 0; k < pl.chain.np; ++k)
+        if (pl.chain.pass[k].gslot) {
+            pl.chain.np = k;
+            break;
+        }
+    pl.fanout = 1;
+    assign_slots(pl);
+    // per-pair buffers first, then one region for the passes' slots
+    const auto r256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t ops_bytes = op_scratch_bytes(n, max_len);
+    const size_t list = r256(n * 4);
+    const size_t fixed = r256(ops_bytes) + r256(CTRL_WORDS * 4) + 4 * list + (pl.nb > 1 ? r256((size_t)pl.nb * n * 4) : 0);
+    if (fixed > bytes)
+        return fail(EWFA_ERR_RANGE, "%zu pairs of up to %u bases need %zu bytes of scratch before any pass; the "
+                    "allocation-free call holds %zu", n, max_len, fixed, bytes);
+    char* b = static_cast<char*>(scratch);
+    Scratch x{};
+    size_t off = 0;
+    const auto take = [&](size_t sz) { char* p = b + off; off += r256(sz); return p; };
+    x.ops = reinterpret_cast<uint64_t*>(take(ops_bytes));
+    x.ctrl = reinterpret_cast<uint32_t*>(take(CTRL_WORDS * 4));
+    x.list_a = reinterpret_cast<uint32_t*>(take(n * 4));
+    x.list_b = reinterpret_cast<uint32_t*>(take(n * 4));
+    x.list_fb = reinterpret_cast<uint32_t*>(take(n * 4));
+    x.bound = reinterpret_cast<uint32_t*>(take(n * 4));
+    x.buckets = pl.nb > 1 ? reinterpret_cast<uint32_t*>(take((size_t)pl.nb * n * 4)) : nullptr;
+    x.region = bytes - off;
+    x.bt = reinterpret_cast<uint64_t*>(b + off);
+    x.runs = reinterpret_cast<uint8_t*>(b + off);
+    // each pass gets at most as many persistent blocks as slots fit the region
+    const auto fit = [&](Pass& p) -> int {
+        size_t slot = 0;
+        if (p.reg) slot = runs_slot_of(p);
+        else if (!p.bt_lds) slot = 16u * (size_t)bt_chunks((uint32_t)p.dcap);
+        if (slot == 0) return EWFA_OK;
+        const size_t most = x.region / slot;
+        if (most == 0)
+            return fail(EWFA_ERR_RANGE, "a pass of %d levels needs a %zu-byte backtrace slot; %zu bytes of the "
+                        "allocation-free call's scratch are left for it", p.dcap, slot, x.region);
+        if (p.grid > most) p.grid = (uint32_t)most;
+        return EWFA_OK;
+    };
+    for (int k = 0; k < pl.nb; ++k) CHK(fit(pl.pass[k]));
+    for (int k = 0; k < pl.chain.np; ++k) CHK(fit(pl.chain.pass[k]));
+    hipStream_t s = c->stream;
+    HIPCHK(hipEventRecord(c->ev_in, (hipStream_t)stream));
+    HIPCHK(hipStreamWaitEvent(s, c->ev_in, 0));
+    c->align_ms = c->emit_ms = 0.f;
+    c->ext = &x;
+    const int rc = enqueue_plan(c, pl, d_words, d_pairs, (uint32_t)n, max_len, d_results, d_cigars, s, c->timing);
+    c->ext = nullptr;
+    CHK(rc);
+    HIPCHK(hipStreamSynchronize(s));
+    if (c->timing) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_a0, c->ev_a1) == hipSuccess) c->align_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev_a1, c->ev_e1) == hipSuccess) c->emit_ms = ms;
+    }
+    return EWFA_OK;
+}
 
 int enqueue_align(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t* d_pairs, size_t n, uint32_t max_len,
                   ewfa_result_t* d_results, int8_t* d_cigars, uint64_t* d_ops, size_t ops_bytes) {
@@ -1204,6 +1284,9 @@ int ewfa_create(ewfa_ctx_t** out, const ewfa_opts_t* opts) {
                 rc = fail(EWFA_ERR_HIP, "aux stream or event creation failed");
     }
     if (rc == EWFA_OK) rc = c->ctrl.reserve(c, CTRL_WORDS * 4);
+    // the allocation-free entry point's scratch address on this device (its
+    // call retries and reports a failure)
+    if (rc == EWFA_OK) (void)ewfa::noalloc_setup(c->device);
     if (rc == EWFA_OK && hipHostMalloc((void**)&c->host_ctrl, CTRL_WORDS * 4, hipHostMallocDefault) != hipSuccess)
         rc = fail(EWFA_ERR_NOMEM, "hipHostMalloc failed");
     if (rc != EWFA_OK) {

@@ -30,6 +30,21 @@ struct Scratch {
     size_t region;
 };
 
+// ewfa_align_device on the persistent path with every scratch buffer carved
+// out of `scratch` (`bytes` long, 256-byte aligned) and eager launches: no
+// device or host memory is allocated and no graph is captured.  Pass grids
+// are narrowed so that their backtrace and run-row slots fit the region left
+// after the per-pair buffers; EWFA_ERR_RANGE when not even one slot does.
+int align_device_in(ewfa_ctx_t* ctx, const uint64_t* d_words, const ewfa_pair_t* d_pairs, size_t n, uint32_t max_len,
+                    ewfa_result_t* d_results, int8_t* d_cigars, void* stream, void* scratch, size_t bytes);
+
+// The device a context runs on.
+int context_device(ewfa_ctx_t* ctx);
+
+// Sets up the allocation-free entry point's scratch on `device` (the
+// current device): its address, looked up once per device.
+int noalloc_setup(int device);
+
 // Op scratch bytes an align call over n pairs up to max_len bases needs.
 size_t op_scratch_bytes(size_t n, uint32_t max_len);
 

@@ -87,8 +87,11 @@ EXPORTS = [
     "ewfa_device_alloc", "ewfa_device_free", "ewfa_memcpy", "ewfa_device_synchronize",
     "ewfa_server_create", "ewfa_server_destroy", "ewfa_server_submit", "ewfa_server_poll", "ewfa_server_wait",
     "ewfa_server_job_latency", "ewfa_server_stats", "ewfa_autotune", "ewfa_host_alloc", "ewfa_host_free",
-    "ewfa_context_memory", "ewfa_policy_read",
+    "ewfa_context_memory", "ewfa_policy_read", "ewfa_align_device_noalloc",
 ]
+
+# device scratch the allocation-free call holds per GPU (EWFA_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 _lib = None
 _vp = ctypes.c_void_p
@@ -111,6 +114,7 @@ def lib():
                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.ewfa_pack_ascii.argtypes = [_vp, _sz, _vp, _sz, _vp, _sz, _vp]
     L.ewfa_align_device.argtypes = [_vp, _vp, _vp, _sz, ctypes.c_uint32, _vp, _vp, _vp]
+    L.ewfa_align_device_noalloc.argtypes = [_vp, _vp, _vp, _sz, ctypes.c_uint32, _vp, _vp, _vp]
     L.ewfa_align_host.argtypes = [_vp, _vp, _sz, _vp, _sz, _vp, _vp, _sz]
     L.ewfa_upload_words.argtypes = [_vp, _vp, _sz]
     L.ewfa_align_resident.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz]
@@ -439,6 +443,12 @@ class Context:
 
     def align_device(self, d_words, d_pairs, n, max_len, d_results, d_cigars, stream=None):
         _check(lib().ewfa_align_device(self.handle, d_words, d_pairs, n, max_len, d_results, d_cigars, stream))
+
+    def align_device_noalloc(self, d_words, d_pairs, n, max_len, d_results, d_cigars, stream=None):
+        """ewfa_align_device_noalloc: the same results with the library's
+        static device scratch and no allocation."""
+        _check(lib().ewfa_align_device_noalloc(self.handle, d_words, d_pairs, n, max_len, d_results, d_cigars,
+                                               stream))
 
     def align_host(self, words, pairs, n, cigar_bytes, results=None, cigars=None):
         """Host arrays in, (results, cigars) host arrays out (fresh arrays, or

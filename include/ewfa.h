@@ -50,7 +50,9 @@ extern "C" {
                                    stream is left empty */
 #define EWFA_PAIR_BADARG     4  /* descriptor invalid (alignment, capacity) */
 #define EWFA_PAIR_TOO_LONG   5  /* plen/tlen above the call's max_len */
-#define EWFA_PAIR_TOO_DISTANT 6 /* distance beyond ewfa_max_distance */
+#define EWFA_PAIR_TOO_DISTANT 6 /* distance beyond ewfa_max_distance (or, in
+                                   the allocation-free call, beyond what LDS
+                                   holds for the pair's length) */
 
 /* One pair: the replacement of WF_element (src/wavefront_structures.h:44-50)
  * plus the addresses PATTERN_PTR/TEXT_PTR and Cigars::get_cigar_ascii derive
@@ -141,6 +143,22 @@ int ewfa_pack_ascii(const char *d_ascii, size_t ascii_stride, const uint32_t *d_
 int ewfa_align_device(ewfa_ctx_t *ctx, const uint64_t *d_words, const ewfa_pair_t *d_pairs,
                       size_t n, uint32_t max_len, ewfa_result_t *d_results, int8_t *d_cigars,
                       void *stream);
+
+/* Allocation-free variant of ewfa_align_device.  It has no counterpart in
+ * the reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results
+ * and streams as ewfa_align_device on a context in the persistent mode; its
+ * kernels are launched eagerly, and all their scratch (op words, lists,
+ * bounds, control words, backtrace slots) is carved from a device array of
+ * EWFA_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on
+ * until the process ends.  The call allocates nothing; a batch whose scratch
+ * does not fit (the op words take ceil((max_len+1)/32)*8 bytes a pair, e.g.
+ * 256 B at 1,000 bp) is refused with EWFA_ERR_RANGE.  Calls that overlap on
+ * one GPU share its array and race: serialise them. */
+#define EWFA_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int ewfa_align_device_noalloc(ewfa_ctx_t *ctx, const uint64_t *d_words, const ewfa_pair_t *d_pairs,
+                              size_t n, uint32_t max_len, ewfa_result_t *d_results, int8_t *d_cigars,
+                              void *stream);
 
 /* Host-buffer pipeline.  Replaces the Sequences batch loop
  * (src/wavefront.cu:173-330, src/aligner.cu:59-66).  Pairs index h_words /
