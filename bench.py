@@ -278,7 +278,8 @@ def leg_served(args, threads, batched_rate):
         f"{k} submissions")
 
     srv = ewfa.Server(slots=args.serve_slots, arena_bytes=args.serve_arena << 30,
-                      max_batch_pairs=args.serve_batch, max_batch_cigar_bytes=args.serve_batch_bytes << 20)
+                      max_batch_pairs=args.serve_batch, max_batch_cigar_bytes=args.serve_batch_bytes << 20,
+                      flags=ewfa.FLAG_SERIAL if args.serve_serial else 0)
     # warm-up: every job once, into its last output set
     warm = [srv.submit(wv, sub, nb, cb, results=outs[-1][0], cigars=outs[-1][1])
             for wv, sub, nb, cb, outs, _ in jobs]
@@ -336,7 +337,8 @@ def leg_served(args, threads, batched_rate):
            "value": round(pairs_done / span, 1), "unit": "alignments/s",
            "frac_of_batched": round(pairs_done / span / batched_rate, 3),
            "launches": int(d("batches")), "peak_batch_jobs": int(st["peak_batch_jobs"]),
-           "slots": args.serve_slots, "max_batch_pairs": args.serve_batch, "throttled_resubmits": throttled,
+           "slots": args.serve_slots, "max_batch_pairs": args.serve_batch, "serial": bool(args.serve_serial),
+           "throttled_resubmits": throttled,
            "arena_peak_bytes": int(st["arena_peak"]), "failed": int(d("failed")),
            "pairs_not_ok": int((res["status"] != ewfa.PAIR_OK).sum()),
            "server_s": {"enqueue": round(d("upload_s"), 3), "device_wait": round(d("align_s"), 3),
@@ -517,6 +519,7 @@ def main():
     ap.add_argument("--serve-ring", type=int, default=6, help="output sets per pool job")
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-batch", type=int, default=131072, help="max pairs per launch")
+    ap.add_argument("--serve-serial", action="store_true", help="each batch's bucket passes on one stream")
     ap.add_argument("--serve-batch-bytes", type=int, default=2048, help="max CIGAR MiB per launch")
     ap.add_argument("--serve-arena", type=int, default=24, help="GiB")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)

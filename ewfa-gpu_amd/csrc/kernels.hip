@@ -870,6 +870,12 @@ struct PairPrefetch {
     int in_regs;
 };
 
+// NT: non-temporal loads, for the multi-chunk kernel, so that the streamed
+// words do not displace its waves' run scratch in L2.  The banded kernel keeps
+// its run rows in LDS and loads normally: the 128-byte line a pair's words
+// share with the next pair's (words are laid out pair after pair, and a wave
+// takes consecutive pairs) then stays in L2 for it.
+template <bool NT>
 __device__ __forceinline__ PairPrefetch prefetch_pair(const AlignArgs& a, uint32_t pidx, int lane) {
     PairPrefetch f;
     f.pr = a.pairs[pidx];
@@ -877,10 +883,8 @@ __device__ __forceinline__ PairPrefetch prefetch_pair(const AlignArgs& a, uint32
     f.in_regs = (nwp + nwt <= 64) ? 1 : 0;
     f.word = 0;
     if (f.in_regs) {
-        // read once: non-temporal, so the streamed words do not displace the
-        // waves' run scratch in L2
-        if (lane < nwp) f.word = __builtin_nontemporal_load(a.words + f.pr.pattern_off + lane);
-        else if (lane < nwp + nwt) f.word = __builtin_nontemporal_load(a.words + f.pr.text_off + (lane - nwp));
+        const uint64_t* g = lane < nwp ? a.words + f.pr.pattern_off + lane : a.words + f.pr.text_off + (lane - nwp);
+        if (lane < nwp + nwt) f.word = NT ? __builtin_nontemporal_load(g) : *g;
     }
     return f;
 }
@@ -1210,7 +1214,7 @@ __global__ __launch_bounds__(64, NC <= 3 ? 8 : 6) void align_reg_kernel(AlignArg
         const uint32_t item = blockIdx.x;
         if (item < count) {
             const uint32_t pidx = a.list ? a.list[item] : item;
-            const PairPrefetch f = prefetch_pair(a, pidx, lane);
+            const PairPrefetch f = prefetch_pair<(NC > 1)>(a, pidx, lane);
             align_pair_reg<NC>(a, pidx, f, lds, lane);
         }
         return;
@@ -1222,12 +1226,12 @@ __global__ __launch_bounds__(64, NC <= 3 ? 8 : 6) void align_reg_kernel(AlignArg
         if (base >= count) break;
         const uint32_t end = min(base + a.grab, count);
         uint32_t pidx = a.list ? a.list[base] : base;
-        PairPrefetch cur = prefetch_pair(a, pidx, lane);
+        PairPrefetch cur = prefetch_pair<(NC > 1)>(a, pidx, lane);
         for (uint32_t it = base; it < end; ++it) {
             const bool more = it + 1 < end;
             const uint32_t nidx = more ? (a.list ? a.list[it + 1] : it + 1) : 0u;
             PairPrefetch nxt = cur;
-            if (more) nxt = prefetch_pair(a, nidx, lane);  // in flight during this pair
+            if (more) nxt = prefetch_pair<(NC > 1)>(a, nidx, lane);  // in flight during this pair
             align_pair_reg<NC>(a, pidx, cur, lds, lane);
             group_sync<1>();
             cur = nxt;

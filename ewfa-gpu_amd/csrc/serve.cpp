@@ -472,9 +472,8 @@ int slot_init(ewfa_server* s, Slot* sl) {
     o.policy_path = s->policy.empty() ? nullptr : s->policy.c_str();
     int rc = ewfa_create(&sl->ctx, &o);
     if (rc != EWFA_OK) return rc;
-    // eager launches (each batch differs); a batch's bucket passes stay on
-    // the slot's stream, the slots themselves running side by side
-    ewfa_set_flags(sl->ctx, EWFA_FLAG_NO_GRAPH | EWFA_FLAG_SERIAL);
+    // eager launches (each batch differs)
+    ewfa_set_flags(sl->ctx, EWFA_FLAG_NO_GRAPH | (s->opts.flags & EWFA_FLAG_SERIAL));
     sl->cs = ewfa::context_stream(sl->ctx);
     if (hipSetDevice(s->opts.device) != hipSuccess ||
         hipStreamCreateWithFlags(&sl->up, hipStreamNonBlocking) != hipSuccess)
@@ -508,6 +507,10 @@ int ewfa_server_create(ewfa_server_t** out, const ewfa_server_opts_t* opts) {
     ewfa_server* s = new ewfa_server();
     if (opts) s->opts = *opts;
     if (s->opts.slots <= 0) s->opts.slots = 2;
+    if (s->opts.flags & ~EWFA_FLAG_SERIAL) {
+        delete s;
+        return EWFA_ERR_ARG;
+    }
     if (s->opts.slots > 64) s->opts.slots = 64;
     if (s->opts.arena_bytes == 0) s->opts.arena_bytes = (size_t)4 << 30;
     if (s->opts.max_batch_pairs == 0) s->opts.max_batch_pairs = (size_t)1 << 17;

@@ -39,7 +39,7 @@ std::vector<ewfa::BucketPolicy> candidates(uint32_t blen) {
     // one wave per pair, wavefront in registers (levels up to 32*chunks-1)
     for (int dcap : {31, 63, 95, 127}) {
         if (dcap > dmax && dcap != 31) continue;
-        for (int grab : {1, 4, 16}) c.push_back({0xFFFFFFFFu, 1, dcap, 1, grab});
+        for (int grab : {1, 4, 16, 64}) c.push_back({0xFFFFFFFFu, 1, dcap, 1, grab});
     }
     // one wave per pair, offsets in LDS
     if (blen >= 400) c.push_back({0xFFFFFFFFu, 1, std::min(255, dmax), 1, 4});

@@ -58,7 +58,8 @@ class BucketPolicy(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("arena_bytes", ctypes.c_size_t),
                 ("max_distance", ctypes.c_int), ("policy_path", ctypes.c_char_p),
-                ("max_batch_pairs", ctypes.c_size_t), ("max_batch_cigar_bytes", ctypes.c_size_t)]
+                ("max_batch_pairs", ctypes.c_size_t), ("max_batch_cigar_bytes", ctypes.c_size_t),
+                ("flags", ctypes.c_int)]
 
 
 class ServerStats(ctypes.Structure):
@@ -522,11 +523,11 @@ class Server:
     launches, admitted first-come first-served by arena footprint."""
 
     def __init__(self, device=0, slots=0, arena_bytes=0, max_distance=0, policy_path=None, max_batch_pairs=0,
-                 max_batch_cigar_bytes=0):
+                 max_batch_cigar_bytes=0, flags=0):
         self._policy = policy_path.encode() if policy_path else None
         o = ServerOpts(device=device, slots=slots, arena_bytes=arena_bytes, max_distance=max_distance,
                        policy_path=self._policy, max_batch_pairs=max_batch_pairs,
-                       max_batch_cigar_bytes=max_batch_cigar_bytes)
+                       max_batch_cigar_bytes=max_batch_cigar_bytes, flags=flags)
         h = _vp()
         _check(lib().ewfa_server_create(ctypes.byref(h), ctypes.byref(o)))
         self.handle = h

@@ -240,6 +240,10 @@ typedef struct ewfa_server_opts {
     size_t max_batch_cigar_bytes; /* their CIGAR regions in bytes (0: 512 MiB);
                                      each slot keeps two pinned host buffers
                                      of this size for the compacted streams */
+    int flags;               /* EWFA_FLAG_SERIAL: each batch's length-bucket
+                                passes one after another on its slot's stream
+                                (default: spread over the slot's three
+                                streams, as ewfa_align_device does) */
 } ewfa_server_opts_t;
 
 typedef struct ewfa_server_stats {
