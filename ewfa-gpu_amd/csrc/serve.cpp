@@ -1,17 +1,19 @@
 // serve.cpp -- the job server: continuous batching of alignment jobs.
 //
-// Each slot owns a device context, a compute stream and an upload stream, a
-// worker thread and a completion thread, and two "flights" (batches in
-// progress).  The worker takes every queued job that fits the device arena,
-// uploads and rebases them into one descriptor array and enqueues the whole
-// persistent path for the batch (bucketing, align passes, the device-driven
-// deferral chain, emits), then the CIGAR compaction, which writes the written
-// stream bytes straight into the flight's pinned host buffer, and the results
-// copy, all without waiting; then it forms the next batch while the device
-// works.  The completion thread waits for a flight, hands each job its results
-// and streams and frees the batch's arena blocks.  No copy ever waits on a
-// stream other than its own, and the compute stream has the next batch queued
-// behind the current one.
+// A staging thread admits submitted jobs in arrival order by their footprint
+// in the device arena: each admitted job gets a block for its packed words and
+// CIGAR regions, and its words go up at once on the server's upload stream,
+// so uploads run at the arrival rate, beside the device's compute.  Each slot
+// owns a device context (its compute streams), a worker thread, a completion
+// thread and two "flights" (batches in progress).  A worker with a free flight
+// takes the staged jobs first-come first-served, merges their descriptors
+// into one batch (uploaded on the same stream, after every job it holds), and
+// enqueues the whole persistent path for the batch (bucketing, align passes,
+// the device-driven deferral chain, emits), then the CIGAR compaction, which
+// writes the written stream bytes straight into the flight's pinned host
+// buffer, and the results copy, all without waiting.  The completion thread
+// waits for a flight and hands each job its results and streams; a job
+// completes as soon as its own outputs are in place.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
@@ -111,7 +113,7 @@ struct Job {
     uint32_t max_len;
     size_t caps;        // sum of the pairs' cigar_cap
     size_t data_bytes;  // resident block: packed words, then CIGAR streams
-    size_t data_off = SIZE_MAX;
+    size_t data_off = SIZE_MAX;  // its block in the arena, from admission
     Clock::time_point submitted, finished;
     int rc = EWFA_OK;
     bool done = false;
@@ -179,7 +181,7 @@ struct Flight {
 struct Slot {
     ewfa_server* s = nullptr;  // declared in ewfa.h
     ewfa_ctx_t* ctx = nullptr;
-    hipStream_t cs = nullptr, up = nullptr;
+    hipStream_t cs = nullptr;
     Flight flights[2];
     std::mutex m;
     std::condition_variable cv;
@@ -197,7 +199,13 @@ struct ewfa_server {
     Arena arena;
     std::mutex m;
     std::condition_variable cv_work, cv_done;
-    std::deque<Job*> queue;
+    // submitted jobs waiting for admission, and admitted (staged) jobs whose
+    // words are uploaded or on their way, waiting for a batch
+    std::deque<Job*> incoming, ready;
+    size_t staged_bytes = 0, stage_cap = 0;
+    int staging = 0;  // jobs taken from `incoming` whose upload is being issued
+    hipStream_t up = nullptr;  // every upload: job words, batch descriptors
+    std::thread stager;
     std::unordered_map<uint64_t, Job*> jobs;
     std::vector<Slot*> slots;
     uint64_t next_id = 1;
@@ -209,37 +217,54 @@ struct ewfa_server {
 
 namespace {
 
-// Takes jobs from the head of the queue into b while their resident blocks
-// and the batch's growing launch state fit the arena together (first come,
-// first served: a head job that does not fit waits, and so does everything
-// behind it).  Called with the server lock held; false if nothing fit.
+// Takes staged jobs from the head of `ready` into b (first come, first
+// served, up to the batch limits, a single larger job excepted) and places
+// the batch's launch state in the arena; while the state does not fit, the
+// last jobs go back to the head of `ready`.  Called with the server lock
+// held; false when nothing could be taken.
 bool form_batch(ewfa_server* s, Batch& b) {
-    while (!s->queue.empty()) {
-        Job* j = s->queue.front();
+    while (!s->ready.empty()) {
+        Job* j = s->ready.front();
         if (!b.jobs.empty() &&
             (b.n + j->n > s->opts.max_batch_pairs || b.caps + j->caps > s->opts.max_batch_cigar_bytes))
             break;
-        const uint32_t ml = std::max(b.max_len, j->max_len);
-        const size_t st = StateLayout(b.n + j->n, ml).total;
-        const size_t off = s->arena.alloc(j->data_bytes);
-        if (off == SIZE_MAX) break;
-        const size_t probe = s->arena.alloc(st);
-        if (probe == SIZE_MAX) {
-            s->arena.free(off, j->data_bytes);
-            break;
-        }
-        s->arena.free(probe, st);
-        j->data_off = off;
         b.jobs.push_back(j);
         b.n += j->n;
         b.caps += j->caps;
-        b.max_len = ml;
-        s->queue.pop_front();
+        b.max_len = std::max(b.max_len, j->max_len);
+        s->ready.pop_front();
+    }
+    while (!b.jobs.empty()) {
+        b.state = StateLayout(b.n, b.max_len).total;
+        b.state_off = s->arena.alloc(b.state);
+        if (b.state_off != SIZE_MAX) break;
+        Job* j = b.jobs.back();
+        b.jobs.pop_back();
+        s->ready.push_front(j);
+        b.n -= j->n;
+        b.caps -= j->caps;
+        b.max_len = 0;
+        for (const Job* k : b.jobs) b.max_len = std::max(b.max_len, k->max_len);
     }
     if (b.jobs.empty()) return false;
-    b.state = StateLayout(b.n, b.max_len).total;
-    b.state_off = s->arena.alloc(b.state);  // the last probe of this size succeeded
-    return b.state_off != SIZE_MAX;
+    for (const Job* j : b.jobs) s->staged_bytes -= j->data_bytes;
+    return true;
+}
+
+// No batch is on the device and the head job's launch state does not fit
+// beside the staged jobs: the staged jobs behind it give their blocks back
+// (after their uploads land) and return to the head of `incoming`, in order.
+// Called with the server lock held.
+void unstage_behind_head(ewfa_server* s) {
+    (void)hipStreamSynchronize(s->up);
+    while (s->ready.size() > 1) {
+        Job* j = s->ready.back();
+        s->ready.pop_back();
+        s->arena.free(j->data_off, j->data_bytes);
+        s->staged_bytes -= j->data_bytes;
+        j->data_off = SIZE_MAX;
+        s->incoming.push_front(j);
+    }
 }
 
 double since(Clock::time_point& t) {
@@ -254,11 +279,10 @@ double since(Clock::time_point& t) {
         if ((x) != hipSuccess) return EWFA_ERR_HIP; \
     } while (0)
 
-// Enqueues a batch: each job's packed words into its block and the rebased
-// descriptors (sequence offsets in words, CIGAR offsets in bytes, both from
-// the arena base) on the upload stream; then, on the compute stream, the
-// persistent path, the CIGAR compaction and the results copy.  Returns
-// without waiting.
+// Enqueues a batch: the rebased descriptors (sequence offsets in words, CIGAR
+// offsets in bytes, both from the arena base) on the upload stream, behind
+// the jobs' words; then, on the compute stream, the persistent path, the
+// CIGAR compaction and the results copy.  Returns without waiting.
 int enqueue_batch(Slot* sl, Flight* f) {
     ewfa_server* s = sl->s;
     const Batch& b = f->b;
@@ -272,7 +296,6 @@ int enqueue_batch(Slot* sl, Flight* f) {
     }
     size_t k = 0;
     for (const Job* j : b.jobs) {
-        if (j->n_words) QCHK(hipMemcpyAsync(base + j->data_off, j->words, j->n_words * 8, hipMemcpyHostToDevice, sl->up));
         const uint64_t woff = j->data_off / 8;
         const uint64_t coff = j->data_off + words_block(j);
         for (size_t i = 0; i < j->n; ++i) {
@@ -286,8 +309,10 @@ int enqueue_batch(Slot* sl, Flight* f) {
     if (b.n == 0) return EWFA_OK;
     const StateLayout L(b.n, b.max_len);
     char* st = base + b.state_off;
-    QCHK(hipMemcpyAsync(st + L.pairs, f->pairs.p, b.n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, sl->up));
-    QCHK(hipEventRecord(f->uploaded, sl->up));
+    // after every upload the staging thread issued for these jobs on the
+    // same stream: one event covers the batch
+    QCHK(hipMemcpyAsync(st + L.pairs, f->pairs.p, b.n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, s->up));
+    QCHK(hipEventRecord(f->uploaded, s->up));
     QCHK(hipStreamWaitEvent(sl->cs, f->uploaded, 0));
     const ewfa_pair_t* d_pairs = reinterpret_cast<const ewfa_pair_t*>(st + L.pairs);
     ewfa_result_t* d_results = reinterpret_cast<ewfa_result_t*>(st + L.results);
@@ -369,8 +394,24 @@ void worker_main(Slot* sl) {
         {
             std::unique_lock<std::mutex> lk(s->m);
             s->cv_work.wait(lk, [&] {
-                if (s->queue.empty()) return s->stopping;
-                return form_batch(s, f->b);
+                if (s->ready.empty()) return s->stopping && s->incoming.empty() && s->staging == 0;
+                if (form_batch(s, f->b)) return true;
+                if (s->running > 0) return false;  // a batch in flight will free memory
+                // nothing on the device: make room for the head job's state
+                unstage_behind_head(s);
+                if (form_batch(s, f->b)) return true;
+                // its words and its state do not fit the arena side by side
+                Job* j = s->ready.front();
+                s->ready.pop_front();
+                s->arena.free(j->data_off, j->data_bytes);
+                s->staged_bytes -= j->data_bytes;
+                j->rc = EWFA_ERR_NOMEM;
+                j->finished = Clock::now();
+                j->done = true;
+                ++s->completed;
+                ++s->failed;
+                s->cv_done.notify_all();
+                return false;
             });
             if (!f->b.jobs.empty()) {
                 s->running += (int)f->b.jobs.size();
@@ -379,7 +420,7 @@ void worker_main(Slot* sl) {
                 ++s->batches;
             }
         }
-        if (f->b.jobs.empty()) {  // stopping, queue drained
+        if (f->b.jobs.empty()) {  // stopping, every job done
             std::lock_guard<std::mutex> lk(sl->m);
             sl->idle.push_back(f);
             break;
@@ -422,7 +463,7 @@ void completer_main(Slot* sl) {
         if (rc == EWFA_OK && hipEventSynchronize(f->done) != hipSuccess) rc = EWFA_ERR_HIP;
         if (rc != EWFA_OK) {
             // nothing may still use the batch's memory when it is freed
-            (void)hipStreamSynchronize(sl->up);
+            (void)hipStreamSynchronize(s->up);
             (void)hipStreamSynchronize(sl->cs);
         }
         const double al = since(t);
@@ -463,6 +504,52 @@ void completer_main(Slot* sl) {
     }
 }
 
+// Admits submitted jobs in arrival order: a block in the arena for the head
+// job's words and CIGAR regions (while the staged jobs' blocks stay within
+// stage_cap, a lone job excepted), then its words up on the server's upload
+// stream at once.
+void stager_main(ewfa_server* s) {
+    (void)hipSetDevice(s->opts.device);
+    for (;;) {
+        Job* j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->m);
+            s->cv_work.wait(lk, [&] {
+                if (s->incoming.empty()) return s->stopping;
+                Job* h = s->incoming.front();
+                if (s->staged_bytes > 0 && s->staged_bytes + h->data_bytes > s->stage_cap) return false;
+                h->data_off = s->arena.alloc(h->data_bytes);
+                return h->data_off != SIZE_MAX;
+            });
+            if (s->incoming.empty()) break;  // stopping
+            j = s->incoming.front();
+            s->incoming.pop_front();
+            s->staged_bytes += j->data_bytes;
+            ++s->staging;
+        }
+        const bool ok = j->n_words == 0 || hipMemcpyAsync(s->arena.at(j->data_off), j->words, j->n_words * 8,
+                                                          hipMemcpyHostToDevice, s->up) == hipSuccess;
+        {
+            std::lock_guard<std::mutex> lk(s->m);
+            --s->staging;
+            if (ok) {
+                s->ready.push_back(j);
+            } else {
+                (void)hipStreamSynchronize(s->up);
+                s->arena.free(j->data_off, j->data_bytes);
+                s->staged_bytes -= j->data_bytes;
+                j->rc = EWFA_ERR_HIP;
+                j->finished = Clock::now();
+                j->done = true;
+                ++s->completed;
+                ++s->failed;
+                s->cv_done.notify_all();
+            }
+        }
+        s->cv_work.notify_all();
+    }
+}
+
 int slot_init(ewfa_server* s, Slot* sl) {
     ewfa_opts_t o;
     ewfa_default_opts(&o);
@@ -475,9 +562,6 @@ int slot_init(ewfa_server* s, Slot* sl) {
     // eager launches (each batch differs)
     ewfa_set_flags(sl->ctx, EWFA_FLAG_NO_GRAPH | (s->opts.flags & EWFA_FLAG_SERIAL));
     sl->cs = ewfa::context_stream(sl->ctx);
-    if (hipSetDevice(s->opts.device) != hipSuccess ||
-        hipStreamCreateWithFlags(&sl->up, hipStreamNonBlocking) != hipSuccess)
-        return EWFA_ERR_HIP;
     for (Flight& f : sl->flights) {
         if (hipEventCreateWithFlags(&f.uploaded, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&f.done, hipEventDisableTiming) != hipSuccess)
@@ -492,7 +576,6 @@ void slot_release(Slot* sl) {
         if (f.uploaded) (void)hipEventDestroy(f.uploaded);
         if (f.done) (void)hipEventDestroy(f.done);
     }
-    if (sl->up) (void)hipStreamDestroy(sl->up);
     if (sl->ctx) ewfa_destroy(sl->ctx);
 }
 
@@ -522,12 +605,21 @@ int ewfa_server_create(ewfa_server_t** out, const ewfa_server_opts_t* opts) {
         delete s;
         return rc;
     }
+    // staged jobs (uploaded, waiting for a batch) hold at most half the arena;
+    // the rest is for the batches on the device
+    s->stage_cap = s->opts.arena_bytes / 2;
+    if (hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) != hipSuccess) {
+        s->arena.release();
+        delete s;
+        return EWFA_ERR_HIP;
+    }
     for (int i = 0; i < s->opts.slots; ++i) {
         Slot* sl = new Slot();
         sl->s = s;
         sl->init_rc = slot_init(s, sl);  // a failed slot fails its batches
         s->slots.push_back(sl);
     }
+    s->stager = std::thread(stager_main, s);
     for (Slot* sl : s->slots) {
         sl->worker = std::thread(worker_main, sl);
         sl->completer = std::thread(completer_main, sl);
@@ -543,6 +635,7 @@ int ewfa_server_destroy(ewfa_server_t* s) {
         s->stopping = true;
     }
     s->cv_work.notify_all();
+    s->stager.join();
     for (Slot* sl : s->slots) {
         sl->worker.join();
         sl->completer.join();
@@ -550,6 +643,10 @@ int ewfa_server_destroy(ewfa_server_t* s) {
         delete sl;
     }
     for (auto& kv : s->jobs) delete kv.second;
+    if (s->up) {
+        (void)hipStreamSynchronize(s->up);
+        (void)hipStreamDestroy(s->up);
+    }
     s->arena.release();
     delete s;
     return EWFA_OK;
@@ -590,7 +687,7 @@ int ewfa_server_submit(ewfa_server_t* s, const uint64_t* h_words, size_t n_words
         std::lock_guard<std::mutex> lk(s->m);
         j->id = s->next_id++;
         s->jobs[j->id] = j;
-        s->queue.push_back(j);
+        s->incoming.push_back(j);
         ++s->submitted;
         *job_id = j->id;
     }
