@@ -269,6 +269,18 @@ def leg_served(args, threads, batched_rate):
         a = b
     log(f"served: pool of {len(jobs)} jobs, {total} pairs, {args.serve_ring} output sets each, "
         f"ready in {time.time() - t0:.1f}s")
+    # the same pairs as one batch from host memory (words pinned, descriptors
+    # and outputs in ordinary host arrays, as the served jobs have them):
+    # the end-to-end batched rate, beside the resident one
+    res_b = np.zeros(total, dtype=ewfa.RESULT_DTYPE)
+    cig_b = np.zeros(max(w.cigar_bytes, 4), dtype=np.int8)
+    with ewfa.Context(mode=ewfa.MODE_PERSISTENT) as ctx:
+        ctx.align_host(words_pin.array, w.pairs, total, w.cigar_bytes, results=res_b, cigars=cig_b)
+        t1 = time.perf_counter()
+        for _ in range(3):
+            ctx.align_host(words_pin.array, w.pairs, total, w.cigar_bytes, results=res_b, cigars=cig_b)
+        batched_e2e = 3 * total / (time.perf_counter() - t1)
+    log(f"served: the pool as one batch from host memory: {batched_e2e / 1e6:.2f} M aln/s")
     mean_pairs = total / len(jobs)
     lam = args.serve_load * batched_rate / mean_pairs
     k = max(len(jobs), int(args.serve_seconds * lam))
@@ -336,6 +348,8 @@ def leg_served(args, threads, batched_rate):
            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
            "value": round(pairs_done / span, 1), "unit": "alignments/s",
            "frac_of_batched": round(pairs_done / span / batched_rate, 3),
+           "batched_resident": round(batched_rate, 1), "batched_end_to_end": round(batched_e2e, 1),
+           "frac_of_batched_end_to_end": round(pairs_done / span / batched_e2e, 3),
            "launches": int(d("batches")), "peak_batch_jobs": int(st["peak_batch_jobs"]),
            "slots": args.serve_slots, "max_batch_pairs": args.serve_batch, "serial": bool(args.serve_serial),
            "throttled_resubmits": throttled,
@@ -344,7 +358,8 @@ def leg_served(args, threads, batched_rate):
            "server_s": {"enqueue": round(d("upload_s"), 3), "device_wait": round(d("align_s"), 3),
                         "handout": round(d("handout_s"), 3)}}
     out["w"], out["res"], out["cig"] = w, res, cig
-    log(f"served: {out['value'] / 1e6:.2f} M aln/s ({out['frac_of_batched']:.2f} of batched), "
+    log(f"served: {out['value'] / 1e6:.2f} M aln/s ({out['frac_of_batched']:.2f} of batched, "
+        f"{out['frac_of_batched_end_to_end']:.2f} of batched from host memory), "
         f"p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, {out['launches']} launches, "
         f"{throttled} throttled")
     return out

@@ -44,11 +44,14 @@ int fail(int code, const char* fmt, ...) {
         if (r_ != EWFA_OK) return r_; \
     } while (0)
 
-// Device buffer of a context that only grows.  Growing it waits for the
-// work queued on its context's streams (the only streams its buffers are used
-// on), so other contexts on the device keep running, and bumps the context's
-// buffer generation: a captured graph that may hold the old address is
-// captured again.
+// Device buffer of a context that only grows, by at least half again, so
+// that batches of varying sizes settle after a few growths.  Growing it never
+// waits: work already queued may still use the old allocation, which the
+// context keeps (its retired list) until it is destroyed, or until an
+// allocation fails and the context's streams have drained.  (Freeing device
+// memory waits for the whole device, other contexts' work included.)  It
+// bumps the context's buffer generation: a captured graph that may hold the
+// old address is captured again.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -113,6 +116,9 @@ struct ewfa_ctx {
     bool use_graph = true;
     int policy_gen = 0, graph_policy_gen = -1;
     uint64_t buffer_gen = 0;        // bumped whenever one of the buffers is replaced
+    std::vector<void*> retired;       // replaced device buffers, freed when idle
+    size_t retired_bytes = 0;
+    std::vector<void*> retired_host;  // replaced pinned host buffers, likewise
     uint64_t graph_buffer_gen = 0;  // buffer_gen when the graph was captured
     // op scratch supplied by the caller for one call (the job server's arena)
     uint64_t* ops_ext = nullptr;
@@ -143,23 +149,42 @@ struct ewfa_ctx {
 
 namespace {
 
+// Waits for the context's streams, then frees its retired buffers.
+void release_retired(ewfa_ctx* c) {
+    if (c->retired.empty() && c->retired_host.empty()) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipStream_t a : c->aux)
+        if (a) (void)hipStreamSynchronize(a);
+    if (c->up) (void)hipStreamSynchronize(c->up);
+    for (void* q : c->retired) (void)hipFree(q);
+    for (void* q : c->retired_host) (void)hipHostFree(q);
+    c->retired.clear();
+    c->retired_host.clear();
+    c->retired_bytes = 0;
+}
+
 int DevBuf::reserve(ewfa_ctx* c, size_t n) {
     if (n <= bytes) return EWFA_OK;
+    size_t want = p ? std::max(n, bytes + bytes / 2) : n;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, want);
+    if (e != hipSuccess && want > n) {
+        (void)hipGetLastError();
+        want = n;
+        e = hipMalloc(&q, want);
+    }
+    if (e != hipSuccess && !c->retired.empty()) {
+        (void)hipGetLastError();
+        release_retired(c);
+        e = hipMalloc(&q, want);
+    }
+    if (e != hipSuccess) return fail(EWFA_ERR_NOMEM, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
     if (p) {
-        // work queued on the context's streams may still use the old buffer
-        if (c->stream) (void)hipStreamSynchronize(c->stream);
-        if (c->up) (void)hipStreamSynchronize(c->up);
-        (void)hipFree(p);
+        c->retired.push_back(p);
+        c->retired_bytes += bytes;
     }
-    p = nullptr;
-    bytes = 0;
-    if (n == 0) return EWFA_OK;
-    hipError_t e = hipMalloc(&p, n);
-    if (e != hipSuccess) {
-        p = nullptr;
-        return fail(EWFA_ERR_NOMEM, "hipMalloc(%zu): %s", n, hipGetErrorString(e));
-    }
-    bytes = n;
+    p = q;
+    bytes = want;
     ++c->buffer_gen;
     return EWFA_OK;
 }
@@ -867,15 +892,18 @@ struct Chunk {
     uint32_t max_len;
 };
 
-// Pinned host memory the device writes directly: coherent, so kernel stores
-// reach host memory without a cache flush.
-int grow_pinned(void** p, size_t* cap, size_t bytes) {
+// Pinned host memory of a context.  coherent: memory the device writes with
+// its own stores (they reach host memory without a cache flush); copies to
+// and from the other kind run on the DMA engines instead of a copy kernel on
+// the CUs.  A replaced buffer is retired, not freed (freeing pinned memory
+// waits for the whole device).
+int grow_pinned(ewfa_ctx* c, void** p, size_t* cap, size_t bytes, bool coherent) {
     if (bytes <= *cap && *p) return EWFA_OK;
-    if (*p) (void)hipHostFree(*p);
+    if (*p) c->retired_host.push_back(*p);
     *p = nullptr;
     *cap = 0;
     const size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 20);
-    if (hipHostMalloc(p, want, hipHostMallocCoherent) != hipSuccess) {
+    if (hipHostMalloc(p, want, coherent ? hipHostMallocCoherent : hipHostMallocNonCoherent) != hipSuccess) {
         *p = nullptr;
         return fail(EWFA_ERR_NOMEM, "hipHostMalloc(%zu) failed", want);
     }
@@ -922,13 +950,13 @@ int pipeline_enqueue(ewfa_ctx* c, const Chunk& ch, size_t k) {
     HIPCHK(hipStreamWaitEvent(c->stream, c->ev_up[k], 0));
     ewfa_ctx::Stage& st = c->stage[k & 1];
     if (ch.caps > st.dense_cap || !st.dense) {
-        CHK(grow_pinned((void**)&st.dense, &st.dense_cap, ch.caps));
+        CHK(grow_pinned(c, (void**)&st.dense, &st.dense_cap, ch.caps, true));
         void* dp = nullptr;
         HIPCHK(hipHostGetDevicePointer(&dp, st.dense, 0));
         st.dense_dev = static_cast<int8_t*>(dp);
     }
     size_t res_bytes = st.res_cap;
-    CHK(grow_pinned((void**)&st.res, &res_bytes, n * sizeof(ewfa_result_t)));
+    CHK(grow_pinned(c, (void**)&st.res, &res_bytes, n * sizeof(ewfa_result_t), false));
     st.res_cap = res_bytes;
     Plan pl;
     CHK(reserve_scratch(c, n, ch.max_len));
@@ -1302,6 +1330,7 @@ int ewfa_destroy(ewfa_ctx_t* c) {
     if (!c) return EWFA_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    release_retired(c);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->graph2) (void)hipGraphExecDestroy(c->graph2);
     for (hipEvent_t e : {c->ev_in, c->ev_a0, c->ev_a1, c->ev_e1, c->ev_fork})
@@ -1405,7 +1434,7 @@ int ewfa_context_memory(ewfa_ctx_t* c, size_t* bytes) {
                             &c->list_fb, &c->bound, &c->gslot, &c->h_words, &c->h_pairs, &c->h_results,
                             &c->h_cigars, &c->dense, &c->scan})
         b += d->bytes;
-    *bytes = b;
+    *bytes = b + c->retired_bytes;
     return EWFA_OK;
 }
 

@@ -2,13 +2,15 @@
 //
 // A staging thread admits submitted jobs in arrival order by their footprint
 // in the device arena: each admitted job gets a block for its packed words and
-// CIGAR regions, and its words go up at once on the server's upload stream,
-// so uploads run at the arrival rate, beside the device's compute.  Each slot
-// owns a device context (its compute streams), a worker thread, a completion
-// thread and two "flights" (batches in progress).  A worker with a free flight
-// takes the staged jobs first-come first-served, merges their descriptors
-// into one batch (uploaded on the same stream, after every job it holds), and
-// enqueues the whole persistent path for the batch (bucketing, align passes,
+// CIGAR regions, and its words go up at once on the server's upload stream
+// (an event marks the end of each job's upload), so uploads run at the arrival
+// rate, beside the device's compute.  Each slot owns a device context (its
+// compute streams), a worker thread, a completion thread and two "flights"
+// (batches in progress).  A worker with a free flight takes the staged jobs
+// first-come first-served, merges their descriptors into one batch, and, on
+// its compute stream, waits for the batch's last upload (jobs go up in the
+// order they are batched), uploads the descriptors and enqueues the whole
+// persistent path for the batch (bucketing, align passes,
 // the device-driven deferral chain, emits), then the CIGAR compaction, which
 // writes the written stream bytes straight into the flight's pinned host
 // buffer, and the results copy, all without waiting.  The completion thread
@@ -114,6 +116,8 @@ struct Job {
     size_t caps;        // sum of the pairs' cigar_cap
     size_t data_bytes;  // resident block: packed words, then CIGAR streams
     size_t data_off = SIZE_MAX;  // its block in the arena, from admission
+    hipEvent_t up_ev = nullptr;  // recorded after its upload, until its batch waits on it
+    uint64_t stage_seq = 0;      // its place in the upload stream's order
     Clock::time_point submitted, finished;
     int rc = EWFA_OK;
     bool done = false;
@@ -143,25 +147,30 @@ struct Batch {
     size_t state_off = SIZE_MAX, state = 0;
 };
 
-// Pinned host buffer that only grows (never while a copy may use it);
-// coherent, so kernel stores reach host memory without a cache flush.
-template <typename T>
+// Pinned host buffer that only grows (never while a copy may use it), by at
+// least half again.  A replaced buffer is kept until the server is destroyed:
+// freeing pinned memory waits for the whole device.
+// COHERENT: for memory the device writes with its own stores (they reach host
+// memory without a cache flush); copies to and from the other kind run on the
+// DMA engines instead of a copy kernel on the CUs.
+template <typename T, bool COHERENT = false>
 struct Pinned {
     T* p = nullptr;
     size_t cap = 0;
+    std::vector<T*> retired;
     ~Pinned() {
         if (p) (void)hipHostFree(p);
+        for (T* q : retired) (void)hipHostFree(q);
     }
     bool reserve(size_t n) {
         if (n <= cap) return true;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-        const size_t want = std::max<size_t>(n + n / 4, 1024);
-        if (hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocCoherent) != hipSuccess) {
-            p = nullptr;
+        const size_t want = std::max<size_t>(std::max(n, cap + cap / 2), 1024);
+        T* q = nullptr;
+        if (hipHostMalloc((void**)&q, want * sizeof(T), COHERENT ? hipHostMallocCoherent : hipHostMallocNonCoherent) !=
+            hipSuccess)
             return false;
-        }
+        if (p) retired.push_back(p);
+        p = q;
         cap = want;
         return true;
     }
@@ -172,9 +181,9 @@ struct Flight {
     Batch b;
     Pinned<ewfa_pair_t> pairs;      // merged, rebased descriptors
     Pinned<ewfa_result_t> results;  // the batch's results
-    Pinned<int8_t> dense;           // the batch's compacted CIGAR bytes, written by the device
+    Pinned<int8_t, true> dense;     // the batch's compacted CIGAR bytes, written by the device
     int8_t* dense_dev = nullptr;    // the same memory as the device addresses it
-    hipEvent_t uploaded = nullptr, done = nullptr;
+    hipEvent_t done = nullptr;
     int rc = EWFA_OK;
 };
 
@@ -204,7 +213,9 @@ struct ewfa_server {
     std::deque<Job*> incoming, ready;
     size_t staged_bytes = 0, stage_cap = 0;
     int staging = 0;  // jobs taken from `incoming` whose upload is being issued
-    hipStream_t up = nullptr;  // every upload: job words, batch descriptors
+    hipStream_t up = nullptr;  // the jobs' words
+    std::vector<hipEvent_t> ev_pool;  // upload events not held by a job
+    uint64_t stage_seq = 0;
     std::thread stager;
     std::unordered_map<uint64_t, Job*> jobs;
     std::vector<Slot*> slots;
@@ -216,6 +227,13 @@ struct ewfa_server {
 };
 
 namespace {
+
+// Returns a staged job's upload event to the pool.  Called with the server
+// lock held.
+void release_upload_event(ewfa_server* s, Job* j) {
+    if (j->up_ev) s->ev_pool.push_back(j->up_ev);
+    j->up_ev = nullptr;
+}
 
 // Takes staged jobs from the head of `ready` into b (first come, first
 // served, up to the batch limits, a single larger job excepted) and places
@@ -263,6 +281,7 @@ void unstage_behind_head(ewfa_server* s) {
         s->arena.free(j->data_off, j->data_bytes);
         s->staged_bytes -= j->data_bytes;
         j->data_off = SIZE_MAX;
+        release_upload_event(s, j);
         s->incoming.push_front(j);
     }
 }
@@ -278,53 +297,6 @@ double since(Clock::time_point& t) {
     do {                                        \
         if ((x) != hipSuccess) return EWFA_ERR_HIP; \
     } while (0)
-
-// Enqueues a batch: the rebased descriptors (sequence offsets in words, CIGAR
-// offsets in bytes, both from the arena base) on the upload stream, behind
-// the jobs' words; then, on the compute stream, the persistent path, the
-// CIGAR compaction and the results copy.  Returns without waiting.
-int enqueue_batch(Slot* sl, Flight* f) {
-    ewfa_server* s = sl->s;
-    const Batch& b = f->b;
-    char* base = s->arena.at(0);
-    if (!f->pairs.reserve(b.n) || !f->results.reserve(b.n)) return EWFA_ERR_NOMEM;
-    if (b.caps > f->dense.cap || !f->dense_dev) {
-        if (!f->dense.reserve(b.caps)) return EWFA_ERR_NOMEM;
-        void* dp = nullptr;
-        QCHK(hipHostGetDevicePointer(&dp, f->dense.p, 0));
-        f->dense_dev = static_cast<int8_t*>(dp);
-    }
-    size_t k = 0;
-    for (const Job* j : b.jobs) {
-        const uint64_t woff = j->data_off / 8;
-        const uint64_t coff = j->data_off + words_block(j);
-        for (size_t i = 0; i < j->n; ++i) {
-            ewfa_pair_t p = j->pairs[i];
-            p.pattern_off += woff;
-            p.text_off += woff;
-            p.cigar_off += coff;
-            f->pairs.p[k++] = p;
-        }
-    }
-    if (b.n == 0) return EWFA_OK;
-    const StateLayout L(b.n, b.max_len);
-    char* st = base + b.state_off;
-    // after every upload the staging thread issued for these jobs on the
-    // same stream: one event covers the batch
-    QCHK(hipMemcpyAsync(st + L.pairs, f->pairs.p, b.n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, s->up));
-    QCHK(hipEventRecord(f->uploaded, s->up));
-    QCHK(hipStreamWaitEvent(sl->cs, f->uploaded, 0));
-    const ewfa_pair_t* d_pairs = reinterpret_cast<const ewfa_pair_t*>(st + L.pairs);
-    ewfa_result_t* d_results = reinterpret_cast<ewfa_result_t*>(st + L.results);
-    int rc = ewfa::enqueue_align(sl->ctx, reinterpret_cast<const uint64_t*>(base), d_pairs, b.n, b.max_len, d_results,
-                                 reinterpret_cast<int8_t*>(base), reinterpret_cast<uint64_t*>(st + L.ops),
-                                 ewfa::op_scratch_bytes(b.n, b.max_len));
-    if (rc != EWFA_OK) return rc;
-    QCHK(ewfa::launch_compact(d_pairs, d_results, reinterpret_cast<const int8_t*>(base), b.n,
-                              reinterpret_cast<uint64_t*>(st + L.scan), f->dense_dev, sl->cs));
-    QCHK(hipMemcpyAsync(f->results.p, d_results, b.n * sizeof(ewfa_result_t), hipMemcpyDeviceToHost, sl->cs));
-    return EWFA_OK;
-}
 
 // Runs fn(k) for k in [0, count) on up to `threads` threads (the calling one
 // included), in contiguous ranges.
@@ -345,8 +317,70 @@ void parallel_ranges(size_t count, int threads, F&& fn) {
     for (std::thread& th : pool) th.join();
 }
 
-// Threads that hand out one batch: one per 4,096 pairs, at most 8.
+// Threads that rebase or hand out one batch: one per 4,096 pairs, at most 8.
 int handout_threads(size_t pairs) { return (int)std::min<size_t>(8, std::max<size_t>(1, pairs / 4096)); }
+
+// Enqueues a batch on the slot's compute stream: a wait for the batch's last
+// upload (the staging thread issues the jobs' words on one stream in order),
+// the rebased descriptors (sequence offsets in words, CIGAR offsets in bytes,
+// both from the arena base), the persistent path, the CIGAR compaction and
+// the results copy.  Returns without waiting.
+int enqueue_batch_on(Slot* sl, Flight* f) {
+    ewfa_server* s = sl->s;
+    const Batch& b = f->b;
+    char* base = s->arena.at(0);
+    if (!f->pairs.reserve(b.n) || !f->results.reserve(b.n)) return EWFA_ERR_NOMEM;
+    if (b.caps > f->dense.cap || !f->dense_dev) {
+        if (!f->dense.reserve(b.caps)) return EWFA_ERR_NOMEM;
+        void* dp = nullptr;
+        QCHK(hipHostGetDevicePointer(&dp, f->dense.p, 0));
+        f->dense_dev = static_cast<int8_t*>(dp);
+    }
+    const size_t nj = b.jobs.size();
+    std::vector<size_t> first(nj + 1, 0);
+    for (size_t k = 0; k < nj; ++k) first[k + 1] = first[k] + b.jobs[k]->n;
+    parallel_ranges(nj, handout_threads(b.n), [&](size_t k) {
+        const Job* j = b.jobs[k];
+        const uint64_t woff = j->data_off / 8;
+        const uint64_t coff = j->data_off + words_block(j);
+        ewfa_pair_t* out = f->pairs.p + first[k];
+        for (size_t i = 0; i < j->n; ++i) {
+            ewfa_pair_t p = j->pairs[i];
+            p.pattern_off += woff;
+            p.text_off += woff;
+            p.cigar_off += coff;
+            out[i] = p;
+        }
+    });
+    const Job* last = nullptr;
+    for (const Job* j : b.jobs)
+        if (j->up_ev && (!last || j->stage_seq > last->stage_seq)) last = j;
+    if (last) QCHK(hipStreamWaitEvent(sl->cs, last->up_ev, 0));
+    if (b.n == 0) return EWFA_OK;
+    const StateLayout L(b.n, b.max_len);
+    char* st = base + b.state_off;
+    QCHK(hipMemcpyAsync(st + L.pairs, f->pairs.p, b.n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, sl->cs));
+    const ewfa_pair_t* d_pairs = reinterpret_cast<const ewfa_pair_t*>(st + L.pairs);
+    ewfa_result_t* d_results = reinterpret_cast<ewfa_result_t*>(st + L.results);
+    int rc = ewfa::enqueue_align(sl->ctx, reinterpret_cast<const uint64_t*>(base), d_pairs, b.n, b.max_len, d_results,
+                                 reinterpret_cast<int8_t*>(base), reinterpret_cast<uint64_t*>(st + L.ops),
+                                 ewfa::op_scratch_bytes(b.n, b.max_len));
+    if (rc != EWFA_OK) return rc;
+    QCHK(ewfa::launch_compact(d_pairs, d_results, reinterpret_cast<const int8_t*>(base), b.n,
+                              reinterpret_cast<uint64_t*>(st + L.scan), f->dense_dev, sl->cs));
+    QCHK(hipMemcpyAsync(f->results.p, d_results, b.n * sizeof(ewfa_result_t), hipMemcpyDeviceToHost, sl->cs));
+    return EWFA_OK;
+}
+
+// The batch's upload events go back to the pool once the compute stream's
+// wait has been enqueued (a wait holds the record it was given), or when
+// enqueueing failed.
+int enqueue_batch(Slot* sl, Flight* f) {
+    const int rc = enqueue_batch_on(sl, f);
+    std::lock_guard<std::mutex> lk(sl->s->m);
+    for (Job* j : f->b.jobs) release_upload_event(sl->s, j);
+    return rc;
+}
 
 // After the batch's device work (results and compacted CIGAR bytes are in the
 // flight's pinned buffers): each job gets its results and its streams placed
@@ -405,6 +439,7 @@ void worker_main(Slot* sl) {
                 s->ready.pop_front();
                 s->arena.free(j->data_off, j->data_bytes);
                 s->staged_bytes -= j->data_bytes;
+                release_upload_event(s, j);
                 j->rc = EWFA_ERR_NOMEM;
                 j->finished = Clock::now();
                 j->done = true;
@@ -427,7 +462,13 @@ void worker_main(Slot* sl) {
         }
         s->cv_work.notify_all();  // the next head may fit beside this batch
         auto t = Clock::now();
-        f->rc = sl->init_rc != EWFA_OK ? sl->init_rc : enqueue_batch(sl, f);
+        if (sl->init_rc != EWFA_OK) {
+            f->rc = sl->init_rc;
+            std::lock_guard<std::mutex> lk(s->m);
+            for (Job* j : f->b.jobs) release_upload_event(s, j);
+        } else {
+            f->rc = enqueue_batch(sl, f);
+        }
         if (f->rc == EWFA_OK && hipEventRecord(f->done, sl->cs) != hipSuccess) f->rc = EWFA_ERR_HIP;
         const double up = since(t);
         {
@@ -512,6 +553,7 @@ void stager_main(ewfa_server* s) {
     (void)hipSetDevice(s->opts.device);
     for (;;) {
         Job* j = nullptr;
+        hipEvent_t ev = nullptr;
         {
             std::unique_lock<std::mutex> lk(s->m);
             s->cv_work.wait(lk, [&] {
@@ -526,15 +568,25 @@ void stager_main(ewfa_server* s) {
             s->incoming.pop_front();
             s->staged_bytes += j->data_bytes;
             ++s->staging;
+            if (!s->ev_pool.empty()) {
+                ev = s->ev_pool.back();
+                s->ev_pool.pop_back();
+            }
         }
-        const bool ok = j->n_words == 0 || hipMemcpyAsync(s->arena.at(j->data_off), j->words, j->n_words * 8,
-                                                          hipMemcpyHostToDevice, s->up) == hipSuccess;
+        bool ok = ev || hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+        if (!ok) ev = nullptr;
+        ok = ok && (j->n_words == 0 || hipMemcpyAsync(s->arena.at(j->data_off), j->words, j->n_words * 8,
+                                                      hipMemcpyHostToDevice, s->up) == hipSuccess);
+        ok = ok && hipEventRecord(ev, s->up) == hipSuccess;
         {
             std::lock_guard<std::mutex> lk(s->m);
             --s->staging;
             if (ok) {
+                j->up_ev = ev;
+                j->stage_seq = ++s->stage_seq;
                 s->ready.push_back(j);
             } else {
+                if (ev) s->ev_pool.push_back(ev);
                 (void)hipStreamSynchronize(s->up);
                 s->arena.free(j->data_off, j->data_bytes);
                 s->staged_bytes -= j->data_bytes;
@@ -563,9 +615,7 @@ int slot_init(ewfa_server* s, Slot* sl) {
     ewfa_set_flags(sl->ctx, EWFA_FLAG_NO_GRAPH | (s->opts.flags & EWFA_FLAG_SERIAL));
     sl->cs = ewfa::context_stream(sl->ctx);
     for (Flight& f : sl->flights) {
-        if (hipEventCreateWithFlags(&f.uploaded, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&f.done, hipEventDisableTiming) != hipSuccess)
-            return EWFA_ERR_HIP;
+        if (hipEventCreateWithFlags(&f.done, hipEventDisableTiming) != hipSuccess) return EWFA_ERR_HIP;
         sl->idle.push_back(&f);
     }
     return EWFA_OK;
@@ -573,7 +623,6 @@ int slot_init(ewfa_server* s, Slot* sl) {
 
 void slot_release(Slot* sl) {
     for (Flight& f : sl->flights) {
-        if (f.uploaded) (void)hipEventDestroy(f.uploaded);
         if (f.done) (void)hipEventDestroy(f.done);
     }
     if (sl->ctx) ewfa_destroy(sl->ctx);
@@ -647,6 +696,7 @@ int ewfa_server_destroy(ewfa_server_t* s) {
         (void)hipStreamSynchronize(s->up);
         (void)hipStreamDestroy(s->up);
     }
+    for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     s->arena.release();
     delete s;
     return EWFA_OK;

@@ -199,7 +199,10 @@ int ewfa_last_timing(ewfa_ctx_t *ctx, float *align_ms, float *emit_ms);
 int ewfa_set_flags(ewfa_ctx_t *ctx, int flags);
 
 /* Device bytes the context holds now: op scratch, backtrace spill buffer,
- * bucket and deferral lists, control words and the host-pipeline buffers. */
+ * bucket and deferral lists, control words and the host-pipeline buffers.
+ * Buffers grow by at least half again and never wait for the device: one
+ * that is replaced is kept (and counted here) until ewfa_destroy, or until an
+ * allocation fails, since freeing device memory waits for the whole device. */
 int ewfa_context_memory(ewfa_ctx_t *ctx, size_t *bytes);
 
 /* Largest distance the device path resolves for sequences up to max_len:
