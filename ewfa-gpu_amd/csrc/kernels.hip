@@ -504,79 +504,103 @@ __device__ __forceinline__ int extend_le3(const uint32_t* P, const uint32_t* T, 
     return acc;
 }
 
-// One level of the block kernel: the group's waves take chunks of 64
-// flattened diagonals in turn (chunk c of level d: diagonals 64c - d ..
-// 64c + 63 - d) over the indices ilo..ihi.  Without a bound (BOUNDED false)
-// the range is the whole level 0..2d, whose every chunk starts with a lane
-// inside it.
+// One chunk of one level of the block kernel (chunk c of level d: diagonals
+// 64c - d .. 64c + 63 - d), for the lanes inside the indices ilo..ihi.  FULL:
+// every lane of the chunk is inside (no range test, lane 0 stores the
+// ballots).
 // BT_PTR: the op ballots are stored through BT (LDS, or HBM with 64-bit
 // addresses), else through the buffer resource BTs with a 32-bit offset.
+template <bool FULL, bool BT_PTR, bool BOUNDED, typename Off>
+__device__ __forceinline__ void block_chunk(int c, int d, int ilo, int ihi, int lane, const Off* curm1, Off* nxt,
+                                            const uint32_t* P32, const uint32_t* T32, int plen, int tlen,
+                                            uint32_t base, uint64_t* BT, __amdgpu_buffer_rsrc_t BTs) {
+    int k2;
+    bool in;
+    if constexpr (BOUNDED) {
+        const int t = (c << 6) + lane - ilo;
+        in = FULL || (unsigned)t <= (unsigned)(ihi - ilo);
+        k2 = 2 * (t + ilo - d);
+    } else {
+        k2 = 2 * lane + ((c << 7) - 2 * d);  // twice the diagonal 64c + lane - d
+        in = FULL || k2 <= 2 * d;
+    }
+    if (in) {
+        // k2: twice the diagonal.  Candidates from level d-1
+        // (src/kernels.cu:131-138): DEL o(k+1), SUB o(k)+1, INS o(k-1)+1,
+        // doubled; the larger wins, DEL on ties, then SUB
+        // (src/kernels.cu:141-152)
+        const Off* q = curm1 + (k2 >> 1);  // diagonals k-1, k, k+1
+        const int ins = q[0], sub = q[1], del = q[2];
+        const int si = max(sub, ins) + 2;
+        const int val = max(del, si);
+        // op ballots: DEL won; SUB at least INS (the op is DEL when its
+        // DEL bit is set, else SUB when this bit is, else INS)
+        const uint64_t lo = __builtin_amdgcn_ballot_w64(del >= si);
+        const uint64_t hi = __builtin_amdgcn_ballot_w64(sub >= ins);
+        const int v = val - k2;
+        // matched bases, capped at the bases left (none past an end); a
+        // valid offset is never left of either start, and a read past an
+        // end stays inside the slot.  The first step compares 16 bases
+        // (most cells' runs end there), the rest 32 at a time.
+        const int mc = min(2 * plen + k2, 2 * tlen);
+        int acc = med3_i32(eq2_le2(P32, T32, v, val), mc - val, 0);
+        if (acc == 32) {
+            int e;
+            do {
+                e = min(eq2_le3(P32, T32, v + acc, val + acc), mc - val - acc);
+                acc += e;
+            } while (e == 64);
+        }
+        // nxt[k]: k2 is even, so 2*k2 is the byte offset
+        nxt[k2 >> 1] = (Off)(val + acc);
+        // the chunk's first lane in the range stores its ballots: lane 0,
+        // unless a bounded range starts inside the chunk
+        bool first;
+        if constexpr (BOUNDED && !FULL) {
+            const uint64_t ex = __builtin_amdgcn_read_exec();
+            first = __builtin_amdgcn_inverse_ballot_w64(ex & (0ull - ex));
+        } else {
+            first = lane == 0;
+        }
+        if (first) {
+            if constexpr (BT_PTR) {
+                uint64_t* bq = BT + 2 * (size_t)(base + (uint32_t)c);
+                bq[0] = lo;
+                bq[1] = hi;
+            } else {
+                const u32x4 w = {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
+                __builtin_amdgcn_raw_buffer_store_b128(w, BTs, 0, (int)(16u * (base + (uint32_t)c)), 0);
+            }
+        }
+    }
+}
+
+// One level of the block kernel: the group's waves take chunks of 64
+// flattened diagonals in turn over the indices ilo..ihi.  The chunks wholly
+// inside the range go without a range test; only the first (with a bound)
+// and the last can be partial.  Without a bound (BOUNDED false) the range is
+// the whole level 0..2d, whose every chunk starts with a lane inside it.
 template <int G, bool BT_PTR, bool BOUNDED, typename Off>
 __device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, int lane, const Off* cur,
                                             Off* nxt, const uint32_t* P32, const uint32_t* T32, int plen,
                                             int tlen, uint32_t base, uint64_t* BT, __amdgpu_buffer_rsrc_t BTs) {
     const Off* curm1 = cur - 1;
-    for (int c = (BOUNDED ? ilo >> 6 : 0) + wave; c <= (ihi >> 6); c += G) {
-        int k2;
-        bool in;
-        if constexpr (BOUNDED) {
-            const int t = (c << 6) + lane - ilo;
-            in = (unsigned)t <= (unsigned)(ihi - ilo);
-            k2 = 2 * (t + ilo - d);
-        } else {
-            k2 = 2 * lane + ((c << 7) - 2 * d);  // twice the diagonal 64c + lane - d
-            in = k2 <= 2 * d;
-        }
-        if (in) {
-            // k2: twice the diagonal.  Candidates from level d-1
-            // (src/kernels.cu:131-138): DEL o(k+1), SUB o(k)+1, INS o(k-1)+1,
-            // doubled; the larger wins, DEL on ties, then SUB
-            // (src/kernels.cu:141-152)
-            const Off* q = curm1 + (k2 >> 1);  // diagonals k-1, k, k+1
-            const int ins = q[0], sub = q[1], del = q[2];
-            const int si = max(sub, ins) + 2;
-            const int val = max(del, si);
-            // op ballots: DEL won; SUB at least INS (the op is DEL when its
-            // DEL bit is set, else SUB when this bit is, else INS)
-            const uint64_t lo = __builtin_amdgcn_ballot_w64(del >= si);
-            const uint64_t hi = __builtin_amdgcn_ballot_w64(sub >= ins);
-            const int v = val - k2;
-            // matched bases, capped at the bases left (none past an end); a
-            // valid offset is never left of either start, and a read past an
-            // end stays inside the slot.  The first step compares 16 bases
-            // (most cells' runs end there), the rest 32 at a time.
-            const int mc = min(2 * plen + k2, 2 * tlen);
-            int acc = med3_i32(eq2_le2(P32, T32, v, val), mc - val, 0);
-            if (acc == 32) {
-                int e;
-                do {
-                    e = min(eq2_le3(P32, T32, v + acc, val + acc), mc - val - acc);
-                    acc += e;
-                } while (e == 64);
-            }
-            // nxt[k]: k2 is even, so 2*k2 is the byte offset
-            nxt[k2 >> 1] = (Off)(val + acc);
-            // the chunk's first lane in the range stores its ballots: lane 0,
-            // unless a bounded range starts inside the chunk
-            bool first;
-            if constexpr (BOUNDED) {
-                const uint64_t ex = __builtin_amdgcn_read_exec();
-                first = __builtin_amdgcn_inverse_ballot_w64(ex & (0ull - ex));
-            } else {
-                first = lane == 0;
-            }
-            if (first) {
-                if constexpr (BT_PTR) {
-                    uint64_t* bq = BT + 2 * (size_t)(base + (uint32_t)c);
-                    bq[0] = lo;
-                    bq[1] = hi;
-                } else {
-                    const u32x4 w = {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
-                    __builtin_amdgcn_raw_buffer_store_b128(w, BTs, 0, (int)(16u * (base + (uint32_t)c)), 0);
-                }
-            }
-        }
+    const int c0 = BOUNDED ? ilo >> 6 : 0;
+    const int cend = ihi >> 6;  // the last chunk, partial unless ihi ends one
+    // a chunk c with c0 < c < cend (or c0 itself when ilo starts it) is full
+    const int cfull = (BOUNDED && (ilo & 63)) ? c0 + 1 : c0;
+    int c = c0 + wave;
+    if (BOUNDED && c < cfull) {
+        block_chunk<false, BT_PTR, BOUNDED, Off>(c, d, ilo, ihi, lane, curm1, nxt, P32, T32, plen, tlen, base, BT,
+                                                 BTs);
+        c += G;
     }
+    for (; c < cend; c += G)
+        block_chunk<true, BT_PTR, BOUNDED, Off>(c, d, ilo, ihi, lane, curm1, nxt, P32, T32, plen, tlen, base, BT,
+                                                BTs);
+    if (c == cend)
+        block_chunk<false, BT_PTR, BOUNDED, Off>(c, d, ilo, ihi, lane, curm1, nxt, P32, T32, plen, tlen, base, BT,
+                                                 BTs);
 }
 
 // BOUNDED: the pass reads a distance bound per pair (a.bound) and leaves out

@@ -123,17 +123,29 @@ struct Job {
     bool done = false;
 };
 
-size_t words_block(const Job* j) { return Arena::round(j->n_words * 8); }
+// Host-to-device copies below this size run as a copy kernel on the CUs, which
+// waits for CU slots behind the persistent passes (up to a pass's length);
+// from this size on they run on a DMA engine.  A job's words smaller than
+// this go up through a bounce slot of exactly this size.
+constexpr size_t kDmaMin = (size_t)64 << 10;
+constexpr int kBounceSlots = 64;
+
+size_t words_bytes(size_t n_words) {
+    const size_t b = n_words * 8;
+    return Arena::round(b > 0 && b < kDmaMin ? kDmaMin : b);
+}
+size_t words_block(const Job* j) { return words_bytes(j->n_words); }
 
 // Launch state of a batch of n pairs up to max_len bases: merged
 // descriptors, results, the traceback op scratch and the compaction's block
-// sums.
+// sums.  The descriptors and results regions hold at least kDmaMin bytes, the
+// size their copies are padded to.
 struct StateLayout {
     size_t pairs, results, ops, scan, total;
     StateLayout(size_t n, uint32_t max_len) {
         pairs = 0;
-        results = pairs + Arena::round(n * sizeof(ewfa_pair_t));
-        ops = results + Arena::round(n * sizeof(ewfa_result_t));
+        results = pairs + Arena::round(std::max(n * sizeof(ewfa_pair_t), kDmaMin));
+        ops = results + Arena::round(std::max(n * sizeof(ewfa_result_t), kDmaMin));
         scan = ops + Arena::round(ewfa::op_scratch_bytes(n, max_len));
         total = scan + Arena::round(16 * ((n + ewfa::COMPACT_BLOCK - 1) / ewfa::COMPACT_BLOCK));
     }
@@ -214,6 +226,10 @@ struct ewfa_server {
     size_t staged_bytes = 0, stage_cap = 0;
     int staging = 0;  // jobs taken from `incoming` whose upload is being issued
     hipStream_t up = nullptr;  // the jobs' words
+    char* bounce = nullptr;    // kBounceSlots pinned slots of kDmaMin bytes, for small jobs' words
+    hipEvent_t bounce_ev[kBounceSlots] = {};
+    bool bounce_used[kBounceSlots] = {};
+    int bounce_next = 0;  // staging thread only
     std::vector<hipEvent_t> ev_pool;  // upload events not held by a job
     uint64_t stage_seq = 0;
     std::thread stager;
@@ -329,7 +345,12 @@ int enqueue_batch_on(Slot* sl, Flight* f) {
     ewfa_server* s = sl->s;
     const Batch& b = f->b;
     char* base = s->arena.at(0);
-    if (!f->pairs.reserve(b.n) || !f->results.reserve(b.n)) return EWFA_ERR_NOMEM;
+    // both copies padded to the DMA size (the regions hold it)
+    const size_t pairs_copy = std::max(b.n * sizeof(ewfa_pair_t), kDmaMin);
+    const size_t results_copy = std::max(b.n * sizeof(ewfa_result_t), kDmaMin);
+    if (!f->pairs.reserve(pairs_copy / sizeof(ewfa_pair_t) + 1) ||
+        !f->results.reserve(results_copy / sizeof(ewfa_result_t) + 1))
+        return EWFA_ERR_NOMEM;
     if (b.caps > f->dense.cap || !f->dense_dev) {
         if (!f->dense.reserve(b.caps)) return EWFA_ERR_NOMEM;
         void* dp = nullptr;
@@ -359,7 +380,7 @@ int enqueue_batch_on(Slot* sl, Flight* f) {
     if (b.n == 0) return EWFA_OK;
     const StateLayout L(b.n, b.max_len);
     char* st = base + b.state_off;
-    QCHK(hipMemcpyAsync(st + L.pairs, f->pairs.p, b.n * sizeof(ewfa_pair_t), hipMemcpyHostToDevice, sl->cs));
+    QCHK(hipMemcpyAsync(st + L.pairs, f->pairs.p, pairs_copy, hipMemcpyHostToDevice, sl->cs));
     const ewfa_pair_t* d_pairs = reinterpret_cast<const ewfa_pair_t*>(st + L.pairs);
     ewfa_result_t* d_results = reinterpret_cast<ewfa_result_t*>(st + L.results);
     int rc = ewfa::enqueue_align(sl->ctx, reinterpret_cast<const uint64_t*>(base), d_pairs, b.n, b.max_len, d_results,
@@ -368,7 +389,7 @@ int enqueue_batch_on(Slot* sl, Flight* f) {
     if (rc != EWFA_OK) return rc;
     QCHK(ewfa::launch_compact(d_pairs, d_results, reinterpret_cast<const int8_t*>(base), b.n,
                               reinterpret_cast<uint64_t*>(st + L.scan), f->dense_dev, sl->cs));
-    QCHK(hipMemcpyAsync(f->results.p, d_results, b.n * sizeof(ewfa_result_t), hipMemcpyDeviceToHost, sl->cs));
+    QCHK(hipMemcpyAsync(f->results.p, d_results, results_copy, hipMemcpyDeviceToHost, sl->cs));
     return EWFA_OK;
 }
 
@@ -545,6 +566,25 @@ void completer_main(Slot* sl) {
     }
 }
 
+// Issues a job's words on the upload stream: straight from the caller's
+// memory, or, below the DMA size, copied into the next bounce slot (once its
+// previous copy has left it) and uploaded whole into the job's words block.
+bool upload_words(ewfa_server* s, const Job* j) {
+    const size_t b = j->n_words * 8;
+    if (b == 0) return true;
+    char* dst = s->arena.at(j->data_off);
+    if (b >= kDmaMin || !s->bounce)
+        return hipMemcpyAsync(dst, j->words, b, hipMemcpyHostToDevice, s->up) == hipSuccess;
+    const int k = s->bounce_next;
+    s->bounce_next = (k + 1) % kBounceSlots;
+    if (s->bounce_used[k] && hipEventSynchronize(s->bounce_ev[k]) != hipSuccess) return false;
+    char* src = s->bounce + (size_t)k * kDmaMin;
+    memcpy(src, j->words, b);
+    if (hipMemcpyAsync(dst, src, kDmaMin, hipMemcpyHostToDevice, s->up) != hipSuccess) return false;
+    s->bounce_used[k] = true;
+    return hipEventRecord(s->bounce_ev[k], s->up) == hipSuccess;
+}
+
 // Admits submitted jobs in arrival order: a block in the arena for the head
 // job's words and CIGAR regions (while the staged jobs' blocks stay within
 // stage_cap, a lone job excepted), then its words up on the server's upload
@@ -575,8 +615,7 @@ void stager_main(ewfa_server* s) {
         }
         bool ok = ev || hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
         if (!ok) ev = nullptr;
-        ok = ok && (j->n_words == 0 || hipMemcpyAsync(s->arena.at(j->data_off), j->words, j->n_words * 8,
-                                                      hipMemcpyHostToDevice, s->up) == hipSuccess);
+        ok = ok && upload_words(s, j);
         ok = ok && hipEventRecord(ev, s->up) == hipSuccess;
         {
             std::lock_guard<std::mutex> lk(s->m);
@@ -662,6 +701,15 @@ int ewfa_server_create(ewfa_server_t** out, const ewfa_server_opts_t* opts) {
         delete s;
         return EWFA_ERR_HIP;
     }
+    // bounce slots: without them small jobs go up directly (a copy kernel)
+    if (hipHostMalloc((void**)&s->bounce, kBounceSlots * kDmaMin, hipHostMallocNonCoherent) != hipSuccess)
+        s->bounce = nullptr;
+    for (hipEvent_t& e : s->bounce_ev)
+        if (s->bounce && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            e = nullptr;
+            (void)hipHostFree(s->bounce);
+            s->bounce = nullptr;
+        }
     for (int i = 0; i < s->opts.slots; ++i) {
         Slot* sl = new Slot();
         sl->s = s;
@@ -697,6 +745,9 @@ int ewfa_server_destroy(ewfa_server_t* s) {
         (void)hipStreamDestroy(s->up);
     }
     for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->bounce_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->bounce) (void)hipHostFree(s->bounce);
     s->arena.release();
     delete s;
     return EWFA_OK;
@@ -727,7 +778,7 @@ int ewfa_server_submit(ewfa_server_t* s, const uint64_t* h_words, size_t n_words
     j->cigar_bytes = cigar_bytes;
     j->max_len = max_len;
     j->caps = caps;
-    j->data_bytes = Arena::round(n_words * 8) + Arena::round(cigar_bytes);
+    j->data_bytes = words_bytes(n_words) + Arena::round(cigar_bytes);
     if (j->data_bytes + StateLayout(n, max_len).total > s->arena.capacity()) {
         delete j;
         return EWFA_ERR_RANGE;

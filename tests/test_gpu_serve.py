@@ -99,11 +99,14 @@ def test_served_mix_at_full_ranges_every_job_against_the_oracle():
 def footprint(w):
     """Arena bytes of a job alone: resident words and CIGAR regions, plus
     descriptors, results, op scratch and compaction block sums (256-byte
-    blocks)."""
+    blocks; words, descriptors and results of at least 64 KiB, the size the
+    server pads their copies to)."""
     r = lambda x: (x + 255) // 256 * 256
+    dma = 64 << 10
     ops = w.n * ((w.max_len + 1) // 32 + 1) * 8
     scan = 16 * ((w.n + 1023) // 1024)
-    return r(w.n_words * 8) + r(w.cigar_bytes) + r(w.n * 40) + r(w.n * 16) + r(ops) + r(scan)
+    return (r(max(w.n_words * 8, dma)) + r(w.cigar_bytes) + r(max(w.n * 40, dma)) + r(max(w.n * 16, dma)) + r(ops)
+            + r(scan))
 
 
 def test_small_arena_admits_as_memory_frees():
