@@ -533,7 +533,7 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=384)
     ap.add_argument("--serve-ring", type=int, default=6, help="output sets per pool job")
     ap.add_argument("--serve-slots", type=int, default=2)
-    ap.add_argument("--serve-batch", type=int, default=131072, help="max pairs per launch")
+    ap.add_argument("--serve-batch", type=int, default=393216, help="max pairs per launch")
     ap.add_argument("--serve-serial", action="store_true", help="each batch's bucket passes on one stream")
     ap.add_argument("--serve-batch-bytes", type=int, default=2048, help="max CIGAR MiB per launch")
     ap.add_argument("--serve-arena", type=int, default=24, help="GiB")

@@ -684,7 +684,7 @@ int ewfa_server_create(ewfa_server_t** out, const ewfa_server_opts_t* opts) {
     }
     if (s->opts.slots > 64) s->opts.slots = 64;
     if (s->opts.arena_bytes == 0) s->opts.arena_bytes = (size_t)4 << 30;
-    if (s->opts.max_batch_pairs == 0) s->opts.max_batch_pairs = (size_t)1 << 17;
+    if (s->opts.max_batch_pairs == 0) s->opts.max_batch_pairs = (size_t)3 << 17;
     if (s->opts.max_batch_cigar_bytes == 0) s->opts.max_batch_cigar_bytes = (size_t)512 << 20;
     if (s->opts.policy_path) s->policy = s->opts.policy_path;
     s->opts.policy_path = nullptr;

@@ -214,8 +214,10 @@ int ewfa_max_distance(ewfa_ctx_t *ctx, uint32_t max_len);
 /* ---- serving -------------------------------------------------------------- */
 
 /* A job server (an addition; the reference has no serving path) with
- * continuous batching.  Each slot has its own device context, compute and
- * upload streams and two batches in flight.  A slot with a free flight takes
+ * continuous batching.  A staging thread admits submitted jobs in arrival
+ * order and uploads their words at once.  Each slot has its own device
+ * context (compute streams) and two batches in flight.  A slot with a free
+ * flight takes
  * the queued jobs first-come first-served, as many as fit the device arena
  * together (and at most max_batch_pairs pairs and max_batch_cigar_bytes of
  * CIGAR regions, a single larger job excepted), and enqueues them as ONE
@@ -239,7 +241,7 @@ typedef struct ewfa_server_opts {
     size_t arena_bytes;      /* device arena for job data and launch state (0: 4 GiB) */
     int max_distance;        /* as ewfa_opts_t */
     const char *policy_path; /* as ewfa_opts_t */
-    size_t max_batch_pairs;  /* pairs merged into one launch (0: 131072) */
+    size_t max_batch_pairs;  /* pairs merged into one launch (0: 393216) */
     size_t max_batch_cigar_bytes; /* their CIGAR regions in bytes (0: 512 MiB);
                                      each slot keeps two pinned host buffers
                                      of this size for the compacted streams */
