@@ -490,6 +490,19 @@ __device__ __forceinline__ int eq2_le2(const uint32_t* P, const uint32_t* T, int
     return (int)(z & ~1u);
 }
 
+// As eq2_le2 without its clamp: twice the equal leading bases (0..30) when
+// the 16-base windows differ, and 62 when they match (the first set bit of
+// zero is all ones).
+__device__ __forceinline__ int eq2_lez(const uint32_t* P, const uint32_t* T, int v2, int h2) {
+    const uint32_t* p = P + (v2 >> 5);
+    const uint32_t* t = T + (h2 >> 5);
+    const uint32_t x = __builtin_amdgcn_alignbit(p[1], p[0], (uint32_t)v2) ^
+                       __builtin_amdgcn_alignbit(t[1], t[0], (uint32_t)h2);
+    uint32_t z;  // v_ffbl_b32 of zero is 0xffffffff; ctz would let the compiler assume z < 32
+    asm("v_ffbl_b32 %0, %1" : "=v"(z) : "v"(x));
+    return (int)(z & 62u);
+}
+
 // Equal bases from (v, h) up to the first mismatch or either end.  Positions
 // past an end extend by nothing; the reads are clamped into the copies.
 __device__ __forceinline__ int extend_le3(const uint32_t* P, const uint32_t* T, int plen, int tlen, int v, int h) {
@@ -514,22 +527,23 @@ template <bool FULL, bool BT_PTR, bool BOUNDED, typename Off>
 __device__ __forceinline__ void block_chunk(int c, int d, int ilo, int ihi, int lane, const Off* curm1, Off* nxt,
                                             const uint32_t* P32, const uint32_t* T32, int plen, int tlen,
                                             uint32_t base, uint64_t* BT, __amdgpu_buffer_rsrc_t BTs) {
-    int k2;
+    // k2: twice the diagonal, 2*lane + kc (kc uniform); its uses below add
+    // the uniform part to uniform bases, so that each is one per-lane add
+    int kc;
     bool in;
     if constexpr (BOUNDED) {
         const int t = (c << 6) + lane - ilo;
         in = FULL || (unsigned)t <= (unsigned)(ihi - ilo);
-        k2 = 2 * (t + ilo - d);
+        kc = 2 * ((c << 6) - d);
     } else {
-        k2 = 2 * lane + ((c << 7) - 2 * d);  // twice the diagonal 64c + lane - d
-        in = FULL || k2 <= 2 * d;
+        kc = (c << 7) - 2 * d;  // twice the diagonal 64c + lane - d, less 2*lane
+        in = FULL || 2 * lane + kc <= 2 * d;
     }
     if (in) {
-        // k2: twice the diagonal.  Candidates from level d-1
-        // (src/kernels.cu:131-138): DEL o(k+1), SUB o(k)+1, INS o(k-1)+1,
-        // doubled; the larger wins, DEL on ties, then SUB
-        // (src/kernels.cu:141-152)
-        const Off* q = curm1 + (k2 >> 1);  // diagonals k-1, k, k+1
+        // Candidates from level d-1 (src/kernels.cu:131-138): DEL o(k+1),
+        // SUB o(k)+1, INS o(k-1)+1, doubled; the larger wins, DEL on ties,
+        // then SUB (src/kernels.cu:141-152)
+        const Off* q = (curm1 + (kc >> 1)) + lane;  // diagonals k-1, k, k+1
         const int ins = q[0], sub = q[1], del = q[2];
         const int si = max(sub, ins) + 2;
         const int val = max(del, si);
@@ -537,22 +551,24 @@ __device__ __forceinline__ void block_chunk(int c, int d, int ilo, int ihi, int 
         // DEL bit is set, else SUB when this bit is, else INS)
         const uint64_t lo = __builtin_amdgcn_ballot_w64(del >= si);
         const uint64_t hi = __builtin_amdgcn_ballot_w64(sub >= ins);
-        const int v = val - k2;
+        const int v = val - 2 * lane - kc;
         // matched bases, capped at the bases left (none past an end); a
         // valid offset is never left of either start, and a read past an
         // end stays inside the slot.  The first step compares 16 bases
-        // (most cells' runs end there), the rest 32 at a time.
-        const int mc = min(2 * plen + k2, 2 * tlen);
-        int acc = med3_i32(eq2_le2(P32, T32, v, val), mc - val, 0);
-        if (acc == 32) {
+        // (most cells' runs end there): eq2_lez gives 32 for a full match
+        // or more (an unmatched window), and a result of 32 or more is
+        // taken as exactly 32 matched, continued 32 at a time.
+        const int mc = min(2 * plen + 2 * lane + kc, 2 * tlen);
+        int acc = med3_i32(eq2_lez(P32, T32, v, val), mc - val, 0);
+        if (acc >= 32) {
+            acc = 32;
             int e;
             do {
                 e = min(eq2_le3(P32, T32, v + acc, val + acc), mc - val - acc);
                 acc += e;
             } while (e == 64);
         }
-        // nxt[k]: k2 is even, so 2*k2 is the byte offset
-        nxt[k2 >> 1] = (Off)(val + acc);
+        nxt[(kc >> 1) + lane] = (Off)(val + acc);
         // the chunk's first lane in the range stores its ballots: lane 0,
         // unless a bounded range starts inside the chunk
         bool first;
