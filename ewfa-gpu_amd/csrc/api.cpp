@@ -691,8 +691,10 @@ int enqueue_chain(ewfa_ctx* c, const Chain& ch, const uint64_t* words, const ewf
     int cur = 0;
     for (int k = 0; k < ch.np; ++k) {
         const Pass& p = ch.pass[k];
-        HIPCHK(ewfa::launch_fill_u32(ctrl + CTRL_QUEUE, 0u, 1, s));
-        HIPCHK(ewfa::launch_fill_u32(ctrl + ctr[1 - cur], 0u, 1, s));
+        // the pass's queue head, the count of the list it defers into and
+        // its bounds' work counter start at zero (one launch)
+        HIPCHK(ewfa::launch_clear_words(
+            ctrl, (1ull << CTRL_QUEUE) | (1ull << ctr[1 - cur]) | (1ull << CTRL_CHAIN_BQ), s));
         ewfa::AlignArgs a = base_args(c, words, pairs, results, max_len);
         fill_pass_args(a, p, c);
         a.list = lists[cur];
@@ -701,7 +703,6 @@ int enqueue_chain(ewfa_ctx* c, const Chain& ch, const uint64_t* words, const ewf
         a.queue = ctrl + CTRL_QUEUE;
         a.defer_list = lists[1 - cur];
         a.defer_count = ctrl + ctr[1 - cur];
-        HIPCHK(ewfa::launch_fill_u32(ctrl + CTRL_CHAIN_BQ, 0u, 1, s));
         CHK(enqueue_bound(c, a, p, ctrl + CTRL_CHAIN_BQ, n, s));
         CHK(launch_pass(a, p, s));
         ewfa::EmitArgs e{};
@@ -849,8 +850,7 @@ int resolve_deferred(ewfa_ctx* c, int prev_dcap, const uint64_t* words, const ew
         a.queue = ctrl + CTRL_QUEUE;
         a.defer_list = out_list;
         a.defer_count = ctrl + out_ctr;
-        HIPCHK(ewfa::launch_fill_u32(ctrl + CTRL_QUEUE, 0u, 1, s));
-        HIPCHK(ewfa::launch_fill_u32(ctrl + out_ctr, 0u, 1, s));
+        HIPCHK(ewfa::launch_clear_words(ctrl, (1ull << CTRL_QUEUE) | (1ull << out_ctr), s));
         HIPCHK(ewfa::launch_align(a, p.waves, true, p.grid, p.lds, s));
         ewfa::EmitArgs e{};
         e.words = words;

@@ -201,6 +201,8 @@ constexpr int EMIT_LDS_MAX_SW = 64;  // 64 KiB of LDS per block
 hipError_t launch_pack(const char* ascii, size_t ascii_stride, const uint32_t* len, size_t nseq,
                        uint64_t* words, size_t words_stride, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
+// zeroes p[i] for every bit i (< 64) set in mask
+hipError_t launch_clear_words(uint32_t* p, uint64_t mask, hipStream_t stream);
 hipError_t launch_bucketize(const ewfa_pair_t* pairs, uint32_t n, const BucketTable& t, uint32_t* lists,
                             uint32_t* counts, hipStream_t stream);
 // Packs every pair's written CIGAR bytes, in pair order, densely into

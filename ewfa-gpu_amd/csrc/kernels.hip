@@ -1259,6 +1259,9 @@ __global__ __launch_bounds__(64, NC <= 3 ? 8 : 6) void align_reg_kernel(AlignArg
         }
         return;
     }
+    // an empty list (a deferral pass with nothing deferred) ends here, not
+    // on a queue atomic that every block of the grid would wait for in turn
+    if (count == 0) return;
     for (;;) {
         uint32_t b = 0;
         if (lane == 0) b = atomicAdd(a.queue, a.grab);
@@ -1299,6 +1302,7 @@ __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persiste
         if (item < count) align_pair<G, BT_LDS, BOUNDED, Off, GSLOT>(a, a.list ? a.list[item] : item, slot, btg, tid);
         return;
     }
+    if (count == 0) return;  // as in align_reg_kernel
     for (;;) {
         uint32_t base;
         if constexpr (G == 1) {
@@ -1414,6 +1418,7 @@ __global__ __launch_bounds__(64) void bound_kernel(AlignArgs a) {
     extern __shared__ uint64_t lds[];
     const int lane = threadIdx.x;
     const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
+    if (count == 0) return;  // as in align_reg_kernel
     for (;;) {
         uint32_t b = 0;
         if (lane == 0) b = atomicAdd(a.queue, 1u);
@@ -1933,6 +1938,13 @@ __global__ __launch_bounds__(256) void pack_kernel(const char* ascii, size_t asc
     words[gid] = out;
 }
 
+// Zeroes the words i < 64 of p whose bit i of mask is set: the control words
+// a deferral pass starts from, in one launch instead of one per word.
+__global__ __launch_bounds__(64) void clear_words_kernel(uint32_t* p, uint64_t mask) {
+    const uint32_t i = threadIdx.x;
+    if ((mask >> i) & 1u) p[i] = 0u;
+}
+
 __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, uint32_t value, size_t n) {
     const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
     if (i < n) p[i] = value;
@@ -2140,6 +2152,12 @@ hipError_t launch_compact(const ewfa_pair_t* pairs, const ewfa_result_t* results
     hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(256), 0, stream, bsum, boff, (uint32_t)nb);
     hipLaunchKernelGGL(compact_copy_kernel, dim3((unsigned)nb), dim3(256), 0, stream, pairs, results, cigars, n,
                        boff, dense);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_words(uint32_t* p, uint64_t mask, hipStream_t stream) {
+    if (mask == 0) return hipSuccess;
+    hipLaunchKernelGGL(clear_words_kernel, dim3(1), dim3(64), 0, stream, p, mask);
     return hipGetLastError();
 }
 
