@@ -413,9 +413,13 @@ def cli_runs(args, w, threads, tmp):
     out["ingest_pack_s"] = round(f.pack_s, 3)
     ewfa.lib().ewfa_free_pairs(ctypes.byref(f))
     assert rc == 0
+    # the autotuned c2 policy, as the c2 leg uses; the built-in table (tuned
+    # for the mixed-length workload) once more without -p for comparison
     cmd = [ewfa.ALIGNER_PATH, "-f", path, "-n", str(w.n), "-l", "1000", "-b", str(args.batch)]
+    out["policy"] = os.path.relpath(POLICY_C2, ROOT)
     wall = lambda text: float(re.search(r"Aligned executed\. Wall time ([0-9.]+)s", text).group(1))
-    for key, extra in (("no_print", []), ("print", ["-p"])):
+    tuned = ["--policy", POLICY_C2]
+    for key, extra in (("no_print", tuned), ("print", tuned + ["-p"]), ("no_print_builtin_policy", [])):
         sink = os.path.join(tmp, f"out_{key}.txt")
         t0 = time.perf_counter()
         with open(sink, "wb") as so:

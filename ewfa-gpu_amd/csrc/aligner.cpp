@@ -45,6 +45,7 @@ struct Params {
     bool check;
     bool standard;
     bool plain;
+    const char* policy;  // autotuned launch policy file, or NULL
     int max_distance;
     unsigned tid;
     unsigned total;
@@ -83,12 +84,43 @@ struct Bytes {
     }
 };
 
+// A batch's descriptors in pinned host memory (the library copies them to
+// the device by DMA; from ordinary memory the copy goes through a staging
+// buffer in pieces).  Grows only.
+struct PinnedPairs {
+    ewfa_pair_t* p = nullptr;
+    size_t n = 0, cap = 0;
+    PinnedPairs() = default;
+    PinnedPairs(const PinnedPairs&) = delete;
+    PinnedPairs& operator=(const PinnedPairs&) = delete;
+    ~PinnedPairs() {
+        if (p) ewfa_host_free(p);
+    }
+    bool assign(const ewfa_pair_t* first, const ewfa_pair_t* last) {
+        const size_t m = (size_t)(last - first);
+        if (m > cap) {
+            void* q = nullptr;
+            if (ewfa_host_alloc(m * sizeof(ewfa_pair_t), &q) != EWFA_OK) return false;
+            if (p) ewfa_host_free(p);
+            p = static_cast<ewfa_pair_t*>(q);
+            cap = m;
+        }
+        if (m) memcpy(p, first, m * sizeof(ewfa_pair_t));
+        n = m;
+        return true;
+    }
+    ewfa_pair_t* data() { return p; }
+    const ewfa_pair_t* data() const { return p; }
+    ewfa_pair_t& operator[](size_t i) { return p[i]; }
+    const ewfa_pair_t& operator[](size_t i) const { return p[i]; }
+};
+
 struct Batch {
     size_t first = 0;   // global index of the batch's first pair
     size_t count = 0;
     std::vector<ewfa_result_t> results;
     Bytes cigars;
-    std::vector<ewfa_pair_t> pairs;
+    PinnedPairs pairs;
 };
 
 // Host threads that format one batch's text (shared by the -t workers).
@@ -198,6 +230,7 @@ void* worker(void* arg) {
     ewfa_opts_t o;
     ewfa_default_opts(&o);
     o.mode = p.plain ? EWFA_MODE_PLAIN : EWFA_MODE_PERSISTENT;
+    o.policy_path = p.policy;
     o.max_distance = p.max_distance;
     ewfa_ctx_t* ctx = nullptr;
     if (ewfa_create(&ctx, &o) != EWFA_OK) {
@@ -251,7 +284,12 @@ void* worker(void* arg) {
         Batch& b = bufs[cur];
         b.first = skip + off;
         b.count = std::min(batch, todo - off);
-        b.pairs.assign(f.pairs + b.first, f.pairs + b.first + b.count);
+        if (!b.pairs.assign(f.pairs + b.first, f.pairs + b.first + b.count)) {
+            std::lock_guard<std::mutex> lk(g_out_mutex);
+            fprintf(stderr, "%s\n", ewfa_last_error());
+            p.rc = 1;
+            break;
+        }
         for (size_t i = 0; i < b.count; ++i) {
             b.pairs[i].pattern_off -= w0;
             b.pairs[i].text_off -= w0;
@@ -352,6 +390,14 @@ int main(int argc, char** argv) {
     if (batch_size > num_alignments) fatal("batch_size must be <= than the number of alignments.");
     if (num_threads > 4096) fatal("Could not allocate memory.\n");
     if (a.max_distance < 0 || a.max_distance > (1 << 30)) fatal("Invalid max distance.\n");
+    if (a.policy) {
+        // read before the timer, as every worker's context will
+        ewfa_bucket_policy_t table[16];
+        if (ewfa_policy_read(a.policy, table, 16) < 0) {
+            fprintf(stderr, "Can not read the launch policy %s.\n", a.policy);
+            exit(1);
+        }
+    }
 
     ewfa_pairs_file_t f;
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
@@ -400,6 +446,7 @@ int main(int argc, char** argv) {
         p.check = a.check != 0;
         p.standard = a.standard_cigar != 0;
         p.plain = a.plain != 0;
+        p.policy = a.policy;
         p.max_distance = (int)a.max_distance;
         p.tid = i;
         p.total = num_threads;

@@ -727,6 +727,24 @@ int enqueue_chain(ewfa_ctx* c, const Chain& ch, const uint64_t* words, const ewf
 // the deferral chain) as two graphs timed between their launches.
 enum { PLAN_ALL = 3, PLAN_ALIGN = 1, PLAN_EMIT = 2 };
 
+// The streams (and join events) a plan's bucket passes fan out over beyond
+// the context's own, created on the first plan that has more than one
+// bucket: a context that only ever runs one bucket (a CLI run on one read
+// length) never pays for them.
+int ensure_aux(ewfa_ctx* c, int F) {
+    for (int i = 0; i < F - 1; ++i) {
+        if (!c->aux[i] && hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking) != hipSuccess) {
+            c->aux[i] = nullptr;
+            return fail(EWFA_ERR_HIP, "aux stream creation failed");
+        }
+        if (!c->ev_join[i] && hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming) != hipSuccess) {
+            c->ev_join[i] = nullptr;
+            return fail(EWFA_ERR_HIP, "join event creation failed");
+        }
+    }
+    return EWFA_OK;
+}
+
 int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_pair_t* pairs, uint32_t n,
                  uint32_t max_len, ewfa_result_t* results, int8_t* cigars, hipStream_t s, bool timing,
                  int phase = PLAN_ALL) {
@@ -735,6 +753,7 @@ int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_
     uint32_t* lists = sx.buckets;
     const bool bucketed = pl.nb > 1;
     const int F = std::max(1, std::min(pl.fanout, ewfa_ctx::MAX_FANOUT));
+    if (phase & PLAN_ALIGN) CHK(ensure_aux(c, F));
     hipStream_t st[ewfa_ctx::MAX_FANOUT] = {s};
     for (int i = 1; i < F; ++i) st[i] = c->aux[i - 1];
     bool fused = false;
@@ -1306,15 +1325,8 @@ int ewfa_create(ewfa_ctx_t** out, const ewfa_opts_t* opts) {
             hipEventCreate(&c->ev_a1) != hipSuccess || hipEventCreate(&c->ev_e1) != hipSuccess ||
             hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess)
             rc = fail(EWFA_ERR_HIP, "hipEventCreate failed");
-        for (int i = 0; rc == EWFA_OK && i < ewfa_ctx::MAX_FANOUT - 1; ++i)
-            if (hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking) != hipSuccess ||
-                hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming) != hipSuccess)
-                rc = fail(EWFA_ERR_HIP, "aux stream or event creation failed");
     }
     if (rc == EWFA_OK) rc = c->ctrl.reserve(c, CTRL_WORDS * 4);
-    // the allocation-free entry point's scratch address on this device (its
-    // call retries and reports a failure)
-    if (rc == EWFA_OK) (void)ewfa::noalloc_setup(c->device);
     if (rc == EWFA_OK && hipHostMalloc((void**)&c->host_ctrl, CTRL_WORDS * 4, hipHostMallocDefault) != hipSuccess)
         rc = fail(EWFA_ERR_NOMEM, "hipHostMalloc failed");
     if (rc != EWFA_OK) {
@@ -1483,6 +1495,7 @@ int ewfa_align_device(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t*
         Plan pl;
         CHK(plan_persistent(c, max_len, (uint32_t)n, pl));
         CHK(reserve_plan(c, pl, n));
+        CHK(ensure_aux(c, pl.fanout));  // not while a graph is being captured
         const bool timing = c->timing;
         ewfa_ctx::Key key{d_words, d_pairs, d_results, d_cigars, ops_base(c), n, max_len, timing};
         if (c->use_graph) {

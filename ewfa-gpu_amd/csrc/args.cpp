@@ -6,7 +6,7 @@
 // option whose short letter is x; an option that takes a value consumes the
 // next argument whatever it looks like (integers through atoll); unknown
 // arguments are ignored; parsing fails only when a required option is
-// missing.  Four options are additions and come after the reference's six.
+// missing.  Five options are additions and come after the reference's six.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -51,6 +51,8 @@ const Option kOptions[] = {
      's', "standard-cigar", false, ARG_NO_VALUE},
     {"Plain launch", "Use the reference's per-batch launch model instead of the persistent kernel.",
      'P', "plain", false, ARG_NO_VALUE},
+    {"Launch policy", "Policy file written by the autotuner for this workload (default: the built-in table).",
+     'u', "policy", false, ARG_STR},
 };
 constexpr int kNumOptions = (int)(sizeof kOptions / sizeof kOptions[0]);
 
@@ -112,6 +114,7 @@ int ewfa_cli_parse(int argc, char** argv, ewfa_cli_args_t* out) {
         out->check = st[7].parsed;
         out->standard_cigar = st[8].parsed;
         out->plain = st[9].parsed;
+        out->policy = st[10].parsed ? st[10].str_val : nullptr;
     }
     for (int o = 0; o < kNumOptions; ++o)
         if (kOptions[o].required && !st[o].parsed) return 0;

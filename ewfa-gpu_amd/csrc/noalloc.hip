@@ -41,15 +41,6 @@ int scratch_ptr(int device, void** out) {
 
 }  // namespace
 
-namespace ewfa {
-
-int noalloc_setup(int device) {
-    void* p = nullptr;
-    return scratch_ptr(device, &p);
-}
-
-}  // namespace ewfa
-
 extern "C" int ewfa_align_device_noalloc(ewfa_ctx_t* ctx, const uint64_t* d_words, const ewfa_pair_t* d_pairs,
                                          size_t n, uint32_t max_len, ewfa_result_t* d_results, int8_t* d_cigars,
                                          void* stream) {

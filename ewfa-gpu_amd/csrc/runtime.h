@@ -41,10 +41,6 @@ int align_device_in(ewfa_ctx_t* ctx, const uint64_t* d_words, const ewfa_pair_t*
 // The device a context runs on.
 int context_device(ewfa_ctx_t* ctx);
 
-// Sets up the allocation-free entry point's scratch on `device` (the
-// current device): its address, looked up once per device.
-int noalloc_setup(int device);
-
 // Op scratch bytes an align call over n pairs up to max_len bases needs.
 size_t op_scratch_bytes(size_t n, uint32_t max_len);
 

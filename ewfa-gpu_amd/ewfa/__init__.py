@@ -47,7 +47,7 @@ class CliArgs(ctypes.Structure):
                 ("batch_size", ctypes.c_int64), ("cpu_threads", ctypes.c_int64),
                 ("batch_set", ctypes.c_int), ("threads_set", ctypes.c_int), ("print_cigars", ctypes.c_int),
                 ("max_distance", ctypes.c_int64), ("check", ctypes.c_int), ("standard_cigar", ctypes.c_int),
-                ("plain", ctypes.c_int)]
+                ("plain", ctypes.c_int), ("policy", ctypes.c_char_p)]
 
 
 class BucketPolicy(ctypes.Structure):

@@ -421,6 +421,8 @@ typedef struct ewfa_cli_args {
     int check;             /* --check (addition) */
     int standard_cigar;    /* --standard-cigar (addition) */
     int plain;             /* --plain (addition): reference launch model */
+    const char *policy;    /* --policy <file> (addition): an autotuned launch
+                              policy; NULL for the built-in table */
 } ewfa_cli_args_t;
 
 /* Returns 1 when every required option was given, 0 otherwise (the caller
