@@ -241,13 +241,14 @@ void* worker(void* arg) {
     const ewfa_pairs_file_t& f = *p.file;
     const uint32_t cap = cigar_capacity(p.seq_len);
     // this worker's pairs read one contiguous range of the packed words (the
-    // reader lays them out pair by pair): it goes up once, and every batch
-    // aligns against it (the reference uploads each batch's sequences,
+    // reader lays them out pair by pair): it goes up once, in pieces that the
+    // batches wait for only as far as they read, and every batch aligns
+    // against it (the reference uploads each batch's sequences,
     // src/wavefront.cu:265-284)
     const uint64_t w0 = f.pairs[skip].pattern_off;
     const ewfa_pair_t& last = f.pairs[skip + todo - 1];
     const uint64_t w1 = last.text_off + ewfa_packed_words(last.tlen);
-    if (ewfa_upload_words(ctx, p.words + w0, (size_t)(w1 - w0)) != EWFA_OK) {
+    if (ewfa_upload_words_async(ctx, p.words + w0, (size_t)(w1 - w0)) != EWFA_OK) {
         std::lock_guard<std::mutex> lk(g_out_mutex);
         fprintf(stderr, "%s\n", ewfa_last_error());
         p.rc = 1;

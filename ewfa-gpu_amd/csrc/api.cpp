@@ -145,6 +145,11 @@ struct ewfa_ctx {
         hipEvent_t done = nullptr;
     } stage[2];
     size_t resident_words = 0;
+    // an ewfa_upload_words_async in progress: its pieces of WORDS_PIECE words
+    // go up on `wup`, piece k ending with ev_words[k]
+    hipStream_t wup = nullptr;
+    std::vector<hipEvent_t> ev_words;
+    size_t words_pieces = 0;
 };
 
 namespace {
@@ -1004,9 +1009,30 @@ int pipeline_complete(ewfa_ctx* c, const Chunk& ch, size_t k, const ewfa_pair_t*
 
 // h_words null: the words of the last ewfa_upload_words are resident.  The
 // descriptors are checked while the chunks are formed.
+// Words per piece of an asynchronous upload (16 MiB).
+constexpr size_t WORDS_PIECE = (size_t)2 << 20;
+
+// Waits for the pieces of an asynchronous words upload still in flight
+// (before anything else writes the context's words).
+int words_settle(ewfa_ctx* c) {
+    if (c->words_pieces) HIPCHK(hipStreamSynchronize(c->wup));
+    c->words_pieces = 0;
+    return EWFA_OK;
+}
+
+// Makes the context's stream wait for the pieces of an asynchronous upload
+// that hold words below w_hi (the pieces land in order).
+int words_wait(ewfa_ctx* c, size_t w_hi) {
+    if (!c->words_pieces || w_hi == 0) return EWFA_OK;
+    const size_t k = std::min((w_hi - 1) / WORDS_PIECE, c->words_pieces - 1);
+    HIPCHK(hipStreamWaitEvent(c->stream, c->ev_words[k], 0));
+    return EWFA_OK;
+}
+
 int run_pipeline(ewfa_ctx* c, const uint64_t* h_words, size_t n_words, const ewfa_pair_t* h_pairs, size_t n,
                  ewfa_result_t* h_results, int8_t* h_cigars, size_t cigar_bytes) {
     if (h_words) {
+        CHK(words_settle(c));
         CHK(c->h_words.reserve(c, std::max<size_t>(n_words, 1) * 8));
         c->resident_words = 0;  // only the words these pairs read go up
     }
@@ -1059,7 +1085,8 @@ int run_pipeline(ewfa_ctx* c, const uint64_t* h_words, size_t n_words, const ewf
     if (rc == EWFA_OK) rc = reserve_scratch(c, most, max_len);
     size_t enq = 0, done = 0;
     for (size_t k = 0; k < chunks.size() && rc == EWFA_OK; ++k) {
-        rc = pipeline_enqueue(c, chunks[k], k);
+        if (!h_words) rc = words_wait(c, chunks[k].w_hi);
+        if (rc == EWFA_OK) rc = pipeline_enqueue(c, chunks[k], k);
         if (rc == EWFA_OK) ++enq;
         if (rc == EWFA_OK && k >= 1) {
             rc = pipeline_complete(c, chunks[k - 1], k - 1, h_pairs, h_results, h_cigars);
@@ -1318,8 +1345,22 @@ int ewfa_create(ewfa_ctx_t** out, const ewfa_opts_t* opts) {
         if (rc == EWFA_ERR_IO) fail(rc, "could not open the policy file %s", c->opts.policy_path);
         if (rc == EWFA_ERR_FORMAT) fail(rc, "bad policy file %s", c->opts.policy_path);
     }
+    // The streams a context's work overlaps on are created together, right
+    // after its compute stream: the runtime deals streams to its (4) hardware
+    // queues by use, so streams created one after another land on different
+    // queues, while one created later, after other contexts' streams, can
+    // share a queue with this context's compute stream, where its work waits
+    // behind the compute (a host-pipeline call ran 51 instead of 68 M aln/s
+    // that way).  With more than one length bucket those are the two streams
+    // the bucket passes fan out over; with one bucket, the upload stream of
+    // the host pipeline (a multi-bucket context creates it on first use).
     if (rc == EWFA_OK && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         rc = fail(EWFA_ERR_HIP, "hipStreamCreate failed");
+    if (rc == EWFA_OK) {
+        if (c->policy.buckets.size() > 1) rc = ensure_aux(c, ewfa_ctx::MAX_FANOUT);
+        else if (hipStreamCreateWithFlags(&c->up, hipStreamNonBlocking) != hipSuccess)
+            rc = fail(EWFA_ERR_HIP, "hipStreamCreate failed");
+    }
     if (rc == EWFA_OK) {
         if (hipEventCreate(&c->ev_in) != hipSuccess || hipEventCreate(&c->ev_a0) != hipSuccess ||
             hipEventCreate(&c->ev_a1) != hipSuccess || hipEventCreate(&c->ev_e1) != hipSuccess ||
@@ -1342,6 +1383,11 @@ int ewfa_destroy(ewfa_ctx_t* c) {
     if (!c) return EWFA_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->wup) {
+        (void)hipStreamSynchronize(c->wup);
+        (void)hipStreamDestroy(c->wup);
+    }
+    for (hipEvent_t e : c->ev_words) (void)hipEventDestroy(e);
     release_retired(c);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->graph2) (void)hipGraphExecDestroy(c->graph2);
@@ -1583,12 +1629,40 @@ int ewfa_upload_words(ewfa_ctx_t* c, const uint64_t* h_words, size_t n_words) {
     if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
     if (!h_words && n_words) return fail(EWFA_ERR_ARG, "null buffer");
     CHK(set_device(c));
+    CHK(words_settle(c));
     CHK(c->h_words.reserve(c, std::max<size_t>(n_words, 1) * 8));
     if (n_words) {
         HIPCHK(hipMemcpyAsync(c->h_words.p, h_words, n_words * 8, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
     c->resident_words = n_words;
+    return EWFA_OK;
+}
+
+int ewfa_upload_words_async(ewfa_ctx_t* c, const uint64_t* h_words, size_t n_words) {
+    if (!c) return fail(EWFA_ERR_ARG, "ctx is NULL");
+    if (!h_words && n_words) return fail(EWFA_ERR_ARG, "null buffer");
+    CHK(set_device(c));
+    CHK(words_settle(c));
+    // work already queued on the context may still read the old words
+    HIPCHK(hipStreamSynchronize(c->stream));
+    CHK(c->h_words.reserve(c, std::max<size_t>(n_words, 1) * 8));
+    c->resident_words = n_words;
+    if (n_words == 0) return EWFA_OK;
+    if (!c->wup) HIPCHK(hipStreamCreateWithFlags(&c->wup, hipStreamNonBlocking));
+    const size_t pieces = (n_words + WORDS_PIECE - 1) / WORDS_PIECE;
+    while (c->ev_words.size() < pieces) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->ev_words.push_back(e);
+    }
+    uint64_t* d = c->h_words.as<uint64_t>();
+    for (size_t k = 0; k < pieces; ++k) {
+        const size_t lo = k * WORDS_PIECE, hi = std::min(n_words, lo + WORDS_PIECE);
+        HIPCHK(hipMemcpyAsync(d + lo, h_words + lo, (hi - lo) * 8, hipMemcpyHostToDevice, c->wup));
+        HIPCHK(hipEventRecord(c->ev_words[k], c->wup));
+        c->words_pieces = k + 1;
+    }
     return EWFA_OK;
 }
 
@@ -1616,6 +1690,7 @@ int ewfa_align_resident(ewfa_ctx_t* c, const ewfa_pair_t* h_pairs, size_t n, ewf
             return fail(EWFA_ERR_ARG, "pair %zu has a bad CIGAR region", i);
     }
     CHK(set_device(c));
+    CHK(words_settle(c));  // the plain mode reads every word
     CHK(c->h_pairs.reserve(c, n * sizeof(ewfa_pair_t)));
     CHK(c->h_results.reserve(c, n * sizeof(ewfa_result_t)));
     CHK(c->h_cigars.reserve(c, std::max<size_t>(cigar_bytes, 4)));

@@ -81,7 +81,7 @@ class PairsFile(ctypes.Structure):
 EXPORTS = [
     "ewfa_default_opts", "ewfa_create", "ewfa_destroy", "ewfa_last_error", "ewfa_device_info",
     "ewfa_device_count", "ewfa_pack_ascii", "ewfa_align_device", "ewfa_align_host", "ewfa_upload_words",
-    "ewfa_align_resident", "ewfa_set_flags", "ewfa_last_timing", "ewfa_max_distance", "ewfa_pack_host",
+    "ewfa_align_resident", "ewfa_upload_words_async", "ewfa_set_flags", "ewfa_last_timing", "ewfa_max_distance", "ewfa_pack_host",
     "ewfa_packed_words", "ewfa_layout_pairs", "ewfa_pack_pairs_host", "ewfa_cigar_text",
     "ewfa_cigar_text_standard", "ewfa_check_cigar", "ewfa_read_pairs", "ewfa_free_pairs",
     "ewfa_synth_max_len", "ewfa_synth_pairs", "ewfa_cli_parse", "ewfa_cli_usage",
@@ -118,6 +118,7 @@ def lib():
     L.ewfa_align_device_noalloc.argtypes = [_vp, _vp, _vp, _sz, ctypes.c_uint32, _vp, _vp, _vp]
     L.ewfa_align_host.argtypes = [_vp, _vp, _sz, _vp, _sz, _vp, _vp, _sz]
     L.ewfa_upload_words.argtypes = [_vp, _vp, _sz]
+    L.ewfa_upload_words_async.argtypes = [_vp, _vp, _sz]
     L.ewfa_align_resident.argtypes = [_vp, _vp, _sz, _vp, _vp, _sz]
     L.ewfa_set_flags.argtypes = [_vp, ctypes.c_int]
     L.ewfa_last_timing.argtypes = [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]

@@ -177,6 +177,19 @@ int ewfa_align_host(ewfa_ctx_t *ctx, const uint64_t *h_words, size_t n_words,
  * the reference's per-batch sequence upload (src/wavefront.cu:66-74, :265-284). */
 int ewfa_upload_words(ewfa_ctx_t *ctx, const uint64_t *h_words, size_t n_words);
 
+/* As ewfa_upload_words, but returns once the copies are queued (an addition;
+ * no reference counterpart, the reference uploads each batch's sequences,
+ * src/wavefront.cu:265-284).  The words go up in pieces of 16 MiB on a
+ * stream of their own, and each ewfa_align_resident call waits only for the
+ * pieces that hold the words its pairs read, so the first batches align
+ * while later words are still on their way.  h_words must stay unchanged
+ * until the last piece has landed: until an ewfa_align_resident call that
+ * reads the last words returns, or the context's next upload or
+ * ewfa_align_host call, or ewfa_destroy.  From pinned memory
+ * (ewfa_host_alloc) the copies are asynchronous; from ordinary memory each
+ * piece is copied before the call returns. */
+int ewfa_upload_words_async(ewfa_ctx_t *ctx, const uint64_t *h_words, size_t n_words);
+
 /* ewfa_align_host against the words of the last ewfa_upload_words. */
 int ewfa_align_resident(ewfa_ctx_t *ctx, const ewfa_pair_t *h_pairs, size_t n,
                         ewfa_result_t *h_results, int8_t *h_cigars, size_t cigar_bytes);

@@ -305,6 +305,42 @@ def test_host_pipeline_several_chunks():
     assert all(np.array_equal(a, b) for a, b in zip(sa[:3000], st))
 
 
+def test_resident_words_uploaded_in_pieces():
+    """ewfa_upload_words_async sends the words in 16 MiB pieces and each
+    ewfa_align_resident call waits only for the pieces its pairs read: the
+    batches, run while later pieces are in flight, equal the host pipeline on
+    every pair; a synchronous upload and a host call afterwards still see
+    their own words."""
+    import ctypes
+    w = workload_case("c2", 40_000, first=3_000)  # about 2.6 M words: two pieces
+    say(f"case resident pieces: {w.n} pairs, {w.n_words} words")
+    with ewfa.Context() as ctx:
+        ref, ref_c = ctx.align_host(w.words, w.pairs, w.n, w.cigar_bytes)
+    pin = ewfa.PinnedArray(w.n_words, np.uint64)
+    pin.array[:] = w.words[:w.n_words]
+    L = ewfa.lib()
+    res = np.zeros(w.n, dtype=ewfa.RESULT_DTYPE)
+    cig = np.zeros(w.cigar_bytes, dtype=np.int8)
+    with ewfa.Context() as ctx:
+        assert L.ewfa_upload_words_async(ctx.handle, pin.ptr, w.n_words) == ewfa.OK
+        for lo, hi in ((0, 7_000), (7_000, 21_000), (21_000, w.n)):
+            sub = np.array(w.pairs[lo:hi])
+            rb = np.zeros(hi - lo, dtype=ewfa.RESULT_DTYPE)
+            assert L.ewfa_align_resident(ctx.handle, sub.ctypes.data, hi - lo, rb.ctypes.data,
+                                         cig.ctypes.data, w.cigar_bytes) == ewfa.OK
+            res[lo:hi] = rb
+        assert np.array_equal(res, ref)
+        sa = ewfa.streams_from(res, w.pairs, cig)
+        sb = ewfa.streams_from(ref, w.pairs, ref_c)
+        assert all(np.array_equal(a, b) for a, b in zip(sa, sb))
+        # a second asynchronous upload, then a host call over other words
+        assert L.ewfa_upload_words_async(ctx.handle, pin.ptr, w.n_words) == ewfa.OK
+        r2, c2 = ctx.align_host(w.words, w.pairs, w.n, w.cigar_bytes)
+        assert np.array_equal(r2, ref)
+    ok, d, sl, st = orc.batch_workload(w, idx=np.arange(0, w.n, 40))
+    assert np.array_equal(res["distance"][::40], d)
+
+
 def _subst(seq, positions):
     out = bytearray(seq)
     for i in positions:
