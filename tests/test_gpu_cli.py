@@ -82,6 +82,37 @@ def test_threads_batches_and_plain(tmp_path):
     assert sorted(cigar_lines(out), key=lambda l: int(l.split(":")[0])) == want
 
 
+def test_policy_option_and_pieced_upload(tmp_path):
+    """--policy with the c2 policy file (one bucket, the banded kernel) and
+    with the mixed-length one gives the same lines as the oracle, over
+    batches whose words arrive in 16 MiB pieces (about 150,000 pairs of
+    1,000 bp read 9.6 M words, five pieces); an unreadable policy file ends
+    the run with status 1."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    pol_dir = os.path.join(root, "ewfa-gpu_amd", "policy")
+    w = ewfa.Workload(ewfa.synth_spec(**ewfa.CONFIGS["c2"]), 150_000, first=11)
+    path = os.path.join(tmp_path, "c2.txt")
+    with open(path, "wb") as f:
+        for i in range(w.n):
+            f.write(b">" + w.pattern(i) + b"\n<" + w.text(i) + b"\n")
+    idx = list(range(0, w.n, 997)) + [w.n - 1]
+    ok, od, osl, ost = orc.batch_workload(w, idx=idx, threads=8)
+    want = {i: f"{i}: {orc.text(ost[j])}" for j, i in enumerate(idx)}
+    for pol in ("c2_1000bp_5pct.policy", "c3_mixed_100_10000bp.policy"):
+        say(f"case cli --policy {pol}: {w.n} pairs, -b 40000")
+        rc, out, err = run(["-f", path, "-n", str(w.n), "-l", "1000", "-b", "40000", "-p",
+                            "--policy", os.path.join(pol_dir, pol)], timeout=300)
+        assert rc == 0, err
+        got = {}
+        for line in out.splitlines():
+            m = re.match(r"^(\d+): ", line)
+            if m and int(m.group(1)) in want:
+                got[int(m.group(1))] = line
+        assert got == want
+    rc, out, err = run(["-f", path, "-n", "10", "-l", "1000", "--policy", os.path.join(tmp_path, "none.policy")])
+    assert rc == 1 and "launch policy" in err
+
+
 def test_without_print_only_two_lines(tmp_path):
     pairs = sample_pairs(50)
     path = pair_file(tmp_path, pairs)

@@ -367,6 +367,15 @@ def test_cli_usage_matches_reference():
     assert len(our_lines) > 7          # then the additions
 
 
+def test_cli_policy_option_parsed():
+    for argv in (["wfe.aligner", "-f", "x", "-n", "3", "-l", "8", "--policy", "p.policy"],
+                 ["wfe.aligner", "-u", "p.policy", "-f", "x", "-n", "3", "-l", "8"]):
+        ok, a = our_parse(argv)
+        assert ok and a.policy == b"p.policy"
+    ok, a = our_parse(["wfe.aligner", "-f", "x", "-n", "3", "-l", "8"])
+    assert ok and a.policy is None
+
+
 @pytest.mark.skipif(ewfa.device_count() > 0, reason="checks the no-device path")
 def test_cli_without_device_fails_cleanly(tmp_path):
     r = subprocess.run([ewfa.ALIGNER_PATH, "-f", "x", "-n", "1", "-l", "1"], capture_output=True, timeout=60)
