@@ -113,6 +113,35 @@ def test_policy_option_and_pieced_upload(tmp_path):
     assert rc == 1 and "launch policy" in err
 
 
+def test_print_to_a_file_matches_the_pipe(tmp_path):
+    """With stdout a regular file the batch text goes out by positional
+    writes from several threads; the file holds the same lines, in the same
+    order, as the piped output (one worker and three), and the wall-time line
+    comes last."""
+    w = ewfa.Workload(ewfa.synth_spec(**ewfa.CONFIGS["c2"]), 30_000, first=77)
+    path = os.path.join(tmp_path, "c2.txt")
+    with open(path, "wb") as f:
+        for i in range(w.n):
+            f.write(b">" + w.pattern(i) + b"\n<" + w.text(i) + b"\n")
+    for extra in ([], ["-t", "3"]):
+        args = ["-f", path, "-n", str(w.n), "-l", "1000", "-b", "7000", "-p"] + extra
+        say(f"case cli -p to a file {extra}")
+        rc, piped, err = run(args, timeout=300)
+        assert rc == 0, err
+        sink = os.path.join(tmp_path, "out.txt")
+        with open(sink, "wb") as so:
+            r = subprocess.run([ewfa.ALIGNER_PATH] + args, stdout=so, stderr=subprocess.PIPE, timeout=300)
+        assert r.returncode == 0, r.stderr.decode()
+        lines = open(sink, "rb").read().decode().splitlines()
+        plines = piped.splitlines()
+        assert lines[0] == plines[0] and lines[-1].startswith("Aligned executed. Wall time")
+        if extra:  # workers interleave whole batches: compare as sets of lines
+            assert sorted(lines[1:-1]) == sorted(plines[1:-1])
+        else:
+            assert lines[1:-1] == plines[1:-1]
+        assert len(lines) == w.n + 2
+
+
 def test_without_print_only_two_lines(tmp_path):
     pairs = sample_pairs(50)
     path = pair_file(tmp_path, pairs)
