@@ -75,7 +75,8 @@ enum : int {
     CTRL_BQUEUE = 8,    // queue heads of the bucket passes (MAX_BUCKETS)
     CTRL_BCOUNT = 16,   // pairs per bucket (MAX_BUCKETS)
     CTRL_BOUNDQ = 24,   // work counters of the bucket passes' distance bounds (MAX_BUCKETS)
-    CTRL_WORDS = 64
+    CTRL_XQ = 64,       // per-XCD work counters of the register bucket passes (MAX_BUCKETS x XCDS, strided)
+    CTRL_WORDS = CTRL_XQ + ewfa::MAX_BUCKETS * ewfa::XCDS * ewfa::XQ_STRIDE
 };
 
 }  // namespace
@@ -764,7 +765,7 @@ int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_
     bool fused = false;
     for (int b = 0; b < pl.nb; ++b) fused = fused || pl.pass[b].reg;
     if (phase & PLAN_ALIGN) {
-        HIPCHK(ewfa::launch_fill_u32(ctrl, 0u, CTRL_BOUNDQ + ewfa::MAX_BUCKETS, s));
+        HIPCHK(ewfa::launch_fill_u32(ctrl, 0u, CTRL_XQ + (size_t)pl.nb * ewfa::XCDS * ewfa::XQ_STRIDE, s));
         if (bucketed) HIPCHK(ewfa::launch_bucketize(pairs, n, pl.table, lists, ctrl + CTRL_BCOUNT, s));
         if (timing) HIPCHK(hipEventRecord(c->ev_a0, s));
         if (F > 1) {
@@ -783,6 +784,7 @@ int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_
             a.count = n;
             a.count_ptr = bucketed ? ctrl + CTRL_BCOUNT + b : nullptr;
             a.queue = ctrl + CTRL_BQUEUE + b;
+            if (p.reg) a.xqueue = ctrl + CTRL_XQ + (size_t)b * ewfa::XCDS * ewfa::XQ_STRIDE;
             a.defer_list = sx.list_a;
             a.defer_count = ctrl + CTRL_DEFER_A;
             a.max_len = pl.blen[b];

@@ -38,6 +38,11 @@ struct BucketPolicy {
 };
 
 constexpr int MAX_BUCKETS = 8;
+// A dispatch's workgroups are dealt to the device's 8 XCDs in turn (block b
+// runs on XCD b % 8).  Per-XCD work counters of a pass sit XQ_STRIDE words
+// (128 bytes, a cache line of their own) apart.
+constexpr int XCDS = 8;
+constexpr int XQ_STRIDE = 32;
 
 // Length limits of the buckets, by value into the bucketing kernel.
 struct BucketTable {
@@ -58,6 +63,9 @@ struct AlignArgs {
     uint32_t count;
     const uint32_t* count_ptr;
     uint32_t* queue;          // persistent: queue head counter (null = plain)
+    // persistent register passes: XCDS counters, XQ_STRIDE words apart, one
+    // per contiguous eighth of the list (null: the one queue above)
+    uint32_t* xqueue;
     uint32_t grab;
     // deferral output (pairs whose distance exceeds dcap)
     uint32_t* defer_list;

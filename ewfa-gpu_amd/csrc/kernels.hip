@@ -1262,12 +1262,9 @@ __global__ __launch_bounds__(64, NC <= 3 ? 8 : 6) void align_reg_kernel(AlignArg
     // an empty list (a deferral pass with nothing deferred) ends here, not
     // on a queue atomic that every block of the grid would wait for in turn
     if (count == 0) return;
-    for (;;) {
-        uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(a.queue, a.grab);
-        const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
-        if (base >= count) break;
-        const uint32_t end = min(base + a.grab, count);
+    // items [base, end), each pair's descriptor and words prefetched while
+    // the pair before aligns
+    const auto run = [&](uint32_t base, uint32_t end) {
         uint32_t pidx = a.list ? a.list[base] : base;
         PairPrefetch cur = prefetch_pair<(NC > 1)>(a, pidx, lane);
         for (uint32_t it = base; it < end; ++it) {
@@ -1280,6 +1277,33 @@ __global__ __launch_bounds__(64, NC <= 3 ? 8 : 6) void align_reg_kernel(AlignArg
             cur = nxt;
             pidx = nidx;
         }
+    };
+    if (a.xqueue) {
+        // the list in XCDS contiguous shares, share q handed out by counter
+        // q: a block takes its own XCD's share first, then helps the others
+        // in turn, so neighbouring pairs (whose descriptors and packed words
+        // share cache lines) mostly go through one XCD's L2
+        const uint32_t home = blockIdx.x % (uint32_t)XCDS;
+        for (uint32_t r = 0; r < (uint32_t)XCDS; ++r) {
+            const uint32_t q = (home + r) % (uint32_t)XCDS;
+            const uint32_t lo = (uint32_t)(((uint64_t)count * q) / XCDS);
+            const uint32_t hi = (uint32_t)(((uint64_t)count * (q + 1)) / XCDS);
+            for (;;) {
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(a.xqueue + q * XQ_STRIDE, a.grab);
+                const uint32_t got = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+                if (got >= hi - lo) break;
+                run(lo + got, min(lo + got + a.grab, hi));
+            }
+        }
+        return;
+    }
+    for (;;) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(a.queue, a.grab);
+        const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+        if (base >= count) break;
+        run(base, min(base + a.grab, count));
     }
 }
 
