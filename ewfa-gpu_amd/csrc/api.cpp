@@ -75,7 +75,7 @@ enum : int {
     CTRL_BQUEUE = 8,    // queue heads of the bucket passes (MAX_BUCKETS)
     CTRL_BCOUNT = 16,   // pairs per bucket (MAX_BUCKETS)
     CTRL_BOUNDQ = 24,   // work counters of the bucket passes' distance bounds (MAX_BUCKETS)
-    CTRL_XQ = 64,       // per-XCD work counters of the register bucket passes (MAX_BUCKETS x XCDS, strided)
+    CTRL_XQ = 64,       // per-XCD work counters of the bucket passes (MAX_BUCKETS x XCDS, strided)
     CTRL_WORDS = CTRL_XQ + ewfa::MAX_BUCKETS * ewfa::XCDS * ewfa::XQ_STRIDE
 };
 
@@ -784,7 +784,7 @@ int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_
             a.count = n;
             a.count_ptr = bucketed ? ctrl + CTRL_BCOUNT + b : nullptr;
             a.queue = ctrl + CTRL_BQUEUE + b;
-            if (p.reg) a.xqueue = ctrl + CTRL_XQ + (size_t)b * ewfa::XCDS * ewfa::XQ_STRIDE;
+            a.xqueue = ctrl + CTRL_XQ + (size_t)b * ewfa::XCDS * ewfa::XQ_STRIDE;
             a.defer_list = sx.list_a;
             a.defer_count = ctrl + CTRL_DEFER_A;
             a.max_len = pl.blen[b];

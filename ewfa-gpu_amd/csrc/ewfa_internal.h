@@ -63,7 +63,7 @@ struct AlignArgs {
     uint32_t count;
     const uint32_t* count_ptr;
     uint32_t* queue;          // persistent: queue head counter (null = plain)
-    // persistent register passes: XCDS counters, XQ_STRIDE words apart, one
+    // persistent bucket passes: XCDS counters, XQ_STRIDE words apart, one
     // per contiguous eighth of the list (null: the one queue above)
     uint32_t* xqueue;
     uint32_t grab;

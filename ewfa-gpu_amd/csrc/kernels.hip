@@ -1327,24 +1327,44 @@ __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persiste
         return;
     }
     if (count == 0) return;  // as in align_reg_kernel
-    for (;;) {
-        uint32_t base;
+    // `grab` items from a work counter, the value seen by the whole group
+    const auto take = [&](uint32_t* counter) -> uint32_t {
         if constexpr (G == 1) {
             uint32_t b = 0;
-            if (tid == 0) b = atomicAdd(a.queue, a.grab);
-            base = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+            if (tid == 0) b = atomicAdd(counter, a.grab);
+            return (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
         } else {
-            if (tid == 0) s_base = atomicAdd(a.queue, a.grab);
+            if (tid == 0) s_base = atomicAdd(counter, a.grab);
             __syncthreads();
-            base = s_base;
+            const uint32_t b = s_base;
             __syncthreads();
+            return b;
         }
-        if (base >= count) break;
-        const uint32_t end = min(base + a.grab, count);
+    };
+    const auto run = [&](uint32_t base, uint32_t end) {
         for (uint32_t it = base; it < end; ++it) {
             align_pair<G, BT_LDS, BOUNDED, Off, GSLOT>(a, a.list ? a.list[it] : it, slot, btg, tid);
             group_sync<G>();  // the slot is reused by the next pair
         }
+    };
+    if (a.xqueue) {  // per-XCD shares, as in align_reg_kernel
+        const uint32_t home = blockIdx.x % (uint32_t)XCDS;
+        for (uint32_t r = 0; r < (uint32_t)XCDS; ++r) {
+            const uint32_t q = (home + r) % (uint32_t)XCDS;
+            const uint32_t lo = (uint32_t)(((uint64_t)count * q) / XCDS);
+            const uint32_t hi = (uint32_t)(((uint64_t)count * (q + 1)) / XCDS);
+            for (;;) {
+                const uint32_t got = take(a.xqueue + q * XQ_STRIDE);
+                if (got >= hi - lo) break;
+                run(lo + got, min(lo + got + a.grab, hi));
+            }
+        }
+        return;
+    }
+    for (;;) {
+        const uint32_t base = take(a.queue);
+        if (base >= count) break;
+        run(base, min(base + a.grab, count));
     }
 }
 
