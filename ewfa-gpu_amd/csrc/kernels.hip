@@ -4,7 +4,8 @@
 //                            NC = 1 is the banded wavefront (one 64-diagonal
 //                            window placed on the target diagonal, passes of
 //                            up to 63 levels), NC = 2..4 the full wavefront in
-//                            NC chunks.  Ops as two 64-bit ballots per level
+//                            NC chunks.  Ops as two bits per lane and level in
+//                            registers (banded) or two 64-bit ballots per level
 //                            and chunk in LDS, traceback, and the CIGAR stream
 //                            written by the same wave (fused emit).
 //   align_kernel<G, BT_LDS>  wavefront search for one pair per group of G
@@ -42,6 +43,19 @@ constexpr int BUF_DWORD3 = 0x00020000;
 __device__ __forceinline__ int writelane_i32(int x, int v, int lane) {
     asm("v_writelane_b32 %0, %1, m0" : "+v"(x) : "s"(v), "{m0}"(lane));
     return x;
+}
+
+// plane with its bits moved up one place and bit 0 set, per lane, where
+// a >= b: a compare into VCC, then plane + plane + carry (v_addc), two VALU
+// instructions.  After n calls bit i holds the compare of the call n-1-i.
+__device__ __forceinline__ uint32_t shift_in_ge(uint32_t plane, int a, int b) {
+    uint32_t r;
+    asm("v_cmp_ge_i32 vcc, %1, %2\n\t"
+        "v_addc_co_u32 %0, vcc, %3, %3, vcc"
+        : "=v"(r)
+        : "v"(a), "v"(b), "v"(plane)
+        : "vcc");
+    return r;
 }
 
 // Median of a, b and c in one instruction.
@@ -986,13 +1000,16 @@ __device__ __forceinline__ int ballot_op32(uint32_t L, uint32_t S, int bit) {
     return op;
 }
 
-// Backtrace and fused emit of a banded pass.  Level l's op ballots sit in
-// lane l-1 of dl/dh (DEL won, diagonal bit k+band of the low/high half) and
-// sl/sh (SUB at least INS; an op is DEL if its DEL bit is set, else SUB if
-// its SUB bit is, else INS).  Status, distance and deferral as finish_pair.
+// Backtrace and fused emit of a banded pass.  Per lane (band lane j holds
+// diagonal j - band), two bit planes per 32 levels: dlo/slo for levels
+// 1..m, m = min(d, 32), level l at bit m - l, and dhi/shi for levels 33..d,
+// level l at bit d - l.  A D bit says DEL won, an S bit SUB at least INS;
+// an op is DEL if its D bit is set, else SUB if its S bit is, else INS.
+// Status, distance and deferral as finish_pair.
 __device__ __forceinline__ void finish_pair_band(const AlignArgs& a, uint32_t pidx, int lane, bool done, int d,
                                                  int limit, int tk, const ewfa_pair_t& pr, uint8_t* R,
-                                                 int band, bool longrun, int dl, int dh, int sl, int sh) {
+                                                 int band, bool longrun, uint32_t dlo, uint32_t slo,
+                                                 uint32_t dhi, uint32_t shi) {
     ewfa_result_t* res = a.results + pidx;
     const int tka = tk < 0 ? -tk : tk;
     const int cap = a.max_distance;
@@ -1021,24 +1038,22 @@ __device__ __forceinline__ void finish_pair_band(const AlignArgs& a, uint32_t pi
         // steps start-1 down to 0 (op s from level s+1) when the target
         // diagonal is inside the last level, else none; each half word fills
         // from its top step down, shifting in two bits a step
-        // the path's diagonal, as a ballot bit (kept in a scalar register)
-        int bit = __builtin_amdgcn_readfirstlane(tk + band);
+        // the path's lane (kept in a scalar register)
+        int pl = __builtin_amdgcn_readfirstlane(tk + band);
+        const int m = min(d, 32);
         for (int s = start - 1; s >= 0;) {
             const int lo = s & ~15;
+            // steps 32 and up (levels 33 and up) are in the upper planes
+            const bool upper = lo >= 32;
+            const uint32_t Dv = upper ? dhi : dlo, Sv = upper ? shi : slo;
+            int pos = (upper ? d : m) - 1 - s;  // level s+1's bit
             uint32_t hw = 0;
-            for (; s >= lo; --s) {
-                // the ballot half that holds the path's bit
-                uint32_t L, S;
-                if (bit >= 32) {
-                    L = (uint32_t)__builtin_amdgcn_readlane(dh, s);
-                    S = (uint32_t)__builtin_amdgcn_readlane(sh, s);
-                } else {
-                    L = (uint32_t)__builtin_amdgcn_readlane(dl, s);
-                    S = (uint32_t)__builtin_amdgcn_readlane(sl, s);
-                }
-                const int op = ballot_op32(L, S, bit);
+            for (; s >= lo; --s, ++pos) {
+                const uint32_t L = (uint32_t)__builtin_amdgcn_readlane((int)Dv, pl);
+                const uint32_t S = (uint32_t)__builtin_amdgcn_readlane((int)Sv, pl);
+                const int op = ballot_op32(L, S, pos);
                 hw = (hw << 2) + (uint32_t)op;
-                bit += op - 2;
+                pl += op - 2;
             }
             opsv = writelane_i32(opsv, (int)hw, lo >> 4);
         }
@@ -1062,8 +1077,9 @@ __device__ __forceinline__ void finish_pair_band(const AlignArgs& a, uint32_t pi
 // lane, and likewise at the top edge), so for a pair that reaches its target
 // within L levels the distance, ops and runs along the path are those of the
 // full wavefront.  A pair that does not is deferred (or capped) exactly as
-// before.  Level d's op ballots go to lane d-1 of four registers, its runs to
-// row d of the run rows in LDS, lane by lane.
+// before.  Level d's op bits are shifted into two bit planes of each lane
+// (four registers over 63 levels), its runs go to row d of the run rows in
+// LDS, lane by lane.
 __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pidx, const PairPrefetch& f,
                                                 uint64_t* slot, int lane) {
     const ewfa_pair_t& pr = f.pr;
@@ -1099,8 +1115,12 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
     if (lane == 0) R[min(max(A, 0), 63)] = (uint8_t)min(o00, 255);
     bool done = (tk == 0) && (o00 == tlen);
     int d = 0;
-    bool longrun = false;  // some level ran a continuation of 7+ steps
-    int dl = 0, dh = 0, sl = 0, sh = 0;  // op ballots, level l in lane l-1
+    // some level ran a continuation of 7+ steps: a run of 255 bases or more
+    // takes 32 + 7 * 32 + more
+    bool longrun = false;
+    int maxit = 0;  // the most continuation steps of a level (uniform)
+    // op bit planes, per lane: levels 1..32 in dlo/slo, 33..63 in dhi/shi
+    uint32_t dlo = 0, slo = 0, dhi = 0, shi = 0;
     if (!done && tka <= limit && limit >= 1) {
         // lanes hold twice the offset (-2: none) and positions are doubled
         // throughout, as eq2_le3 takes them
@@ -1115,8 +1135,11 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
         int sdel = -2, sins = -2;
         uint64_t vmask = 1ull << A;  // lanes of diagonals -d..d, clipped to the window
         int lv = 0;  // this level minus one
-        int stop;
-        do {
+        int stop = 0;  // 64 once the target diagonal holds tlen, else lv
+        // one level, its op bits shifted into the planes dp and sp
+        // wr: this lane's byte in the run row of the next level
+        uint8_t* wr = R + 64 + lane;
+        const auto level = [&](uint32_t& dp, uint32_t& sp) {
             vmask |= (vmask << 1) | (vmask >> 1);
             const bool valid = __builtin_amdgcn_inverse_ballot_w64(vmask);
             // candidates (src/kernels.cu:131-138): DEL from k+1 (lane j+1),
@@ -1129,12 +1152,8 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
             sins = __builtin_amdgcn_update_dpp(sins, o, 0x138, 0xF, 0xF, false);
             const int si = max(o, sins) + 2;
             const int h = max(sdel, si);  // twice the new offset
-            const uint64_t bl = __builtin_amdgcn_ballot_w64(sdel >= si);
-            const uint64_t bs = __builtin_amdgcn_ballot_w64(o >= sins);
-            dl = writelane_i32(dl, (int)(uint32_t)bl, lv);
-            dh = writelane_i32(dh, (int)(uint32_t)(bl >> 32), lv);
-            sl = writelane_i32(sl, (int)(uint32_t)bs, lv);
-            sh = writelane_i32(sh, (int)(uint32_t)(bs >> 32), lv);
+            dp = shift_in_ge(dp, sdel, si);  // DEL won
+            sp = shift_in_ge(sp, o, sins);   // SUB at least INS
             // twice the matched bases, capped at the bases left (none past an
             // end).  Every lane compares: one outside the wavefront holds
             // offset 0, its reads stay inside the slot's guards, and its
@@ -1155,25 +1174,31 @@ __device__ __forceinline__ void align_pair_band(const AlignArgs& a, uint32_t pid
                     cm &= __builtin_amdgcn_ballot_w64(last == 64);
                 } while (cm);
                 acc = hh - h;
-                // a run of 255 bases or more takes 32 + 7 * 32 + more: seven
-                // continuation steps at least
-                longrun |= it >= 7;
+                maxit = max(maxit, it);
             }
             // the run's low byte in row lv+1 (a pair with a run of 255+ is
             // emitted by the emit kernel, which recomputes the runs)
-            R[64 * lv + 64 + lane] = (uint8_t)(acc >> 1);
+            *wr = (uint8_t)(acc >> 1);
+            wr += 64;
             o = valid ? h + acc : -2;
             ++lv;
-            // past the limit (<= 63) once the target diagonal holds tlen
             stop = __builtin_amdgcn_readlane(o, tl) == want ? 64 : lv;
-        } while (stop < limit);
+        };
+        const int lim1 = min(limit, 32);
+        do level(dlo, slo);
+        while (stop < lim1);
+        if (stop < limit) {  // limit <= 63
+            do level(dhi, shi);
+            while (stop < limit);
+        }
         d = lv;
         done = stop == 64;
+        longrun = maxit >= 7;
     }
     group_sync<1>();
     // (when |tk| > limit nothing is walked, and row 0 is read where it was
     // written)
-    finish_pair_band(a, pidx, lane, done, d, limit, tk, pr, R, min(max(A, 0), 63), longrun, dl, dh, sl, sh);
+    finish_pair_band(a, pidx, lane, done, d, limit, tk, pr, R, min(max(A, 0), 63), longrun, dlo, slo, dhi, shi);
 }
 
 // Register-resident wavefront for one wave per pair.  The offsets of a level
