@@ -209,6 +209,7 @@ struct Pass {
     int seq_words;
     int reg;  // register-wavefront chunks (0: offsets staged in LDS)
     int off16;  // 16-bit LDS offsets (block kernel, backtrace in HBM, no bounds)
+    int band_cap;  // bounded pass with narrow slots: the largest bound they hold (0: none)
     size_t lds;
     uint32_t grid;
     bool persistent;
@@ -270,6 +271,26 @@ void choose_off16(Pass& p, uint32_t max_len) {
     p.off16 = (!p.reg && !p.bt_lds && p.dcap < BOUND_MIN_DCAP && (size_t)max_len + (size_t)p.dcap <= 16383) ? 1 : 0;
 }
 
+// A bounded block pass (a budget of BOUND_MIN_DCAP levels or more, whose
+// sequences the bound kernel can stage) whose slot lets fewer blocks share a
+// CU than its narrow slot would: narrow slots sized for one block more per
+// CU.  Such a slot keeps only the diagonals within a pair's bound U, at most
+// U + 1 (against 2 * dcap + 5 cells a generation), and holds bounds up to
+// band_cap; pairs with larger bounds go to the deferral chain.  Only bucket
+// passes take it: the chain's passes must hold every pair they get.
+void choose_band_cap(ewfa_ctx* c, Pass& p) {
+    p.band_cap = 0;
+    if (p.reg || p.gslot || p.bt_lds || p.off16 || p.dcap < BOUND_MIN_DCAP) return;
+    if (ewfa::lds_bytes_bound(p.seq_words) > 65536) return;  // no bounds for this pass
+    const size_t budget = lds_budget(c);
+    const size_t per_cu = budget / ewfa::lds_bytes(p.seq_words, p.dcap, 0, 0);
+    const size_t room = budget / (per_cu + 1);
+    const size_t seq = 16u * (size_t)p.seq_words;
+    if (room < seq + 8u * (size_t)(BOUND_MIN_DCAP + 6)) return;
+    const int cap = (int)(((room & ~(size_t)15) - seq) / 8u) - 6;
+    p.band_cap = std::min(cap, p.dcap);
+}
+
 int finish_pass(ewfa_ctx* c, Pass& p, uint32_t items) {
     if (p.gslot) {
         // slot in HBM, no dynamic LDS; blocks bounded by occupancy and memory
@@ -284,15 +305,16 @@ int finish_pass(ewfa_ctx* c, Pass& p, uint32_t items) {
         p.grid = (uint32_t)std::max<uint64_t>(g, 1);
         return EWFA_OK;
     }
-    p.lds = p.reg ? ewfa::lds_bytes_reg(p.seq_words, p.dcap, p.reg)
-                  : ewfa::lds_bytes(p.seq_words, p.dcap, p.bt_lds, p.off16);
+    p.lds = p.reg        ? ewfa::lds_bytes_reg(p.seq_words, p.dcap, p.reg)
+            : p.band_cap ? ewfa::lds_bytes_narrow(p.seq_words, p.band_cap)
+                         : ewfa::lds_bytes(p.seq_words, p.dcap, p.bt_lds, p.off16);
     if (p.lds > lds_budget(c))
         return fail(EWFA_ERR_RANGE, "pass needs %zu bytes of LDS (device has %zu)", p.lds, c->lds_max);
     // (the block kernels' LDS limit was raised to the budget at ewfa_create)
     if (p.lds > 65536 && p.reg) return fail(EWFA_ERR_RANGE, "register pass above 64 KiB of LDS");
     if (p.persistent) {
         int bpc = p.reg ? ewfa::align_reg_blocks_per_cu(p.reg, p.lds)
-                        : ewfa::align_blocks_per_cu(p.waves, p.lds, p.off16 ? 3 : 1);
+                        : ewfa::align_blocks_per_cu(p.waves, p.lds, p.off16 ? 3 : p.band_cap ? 2 : 1);
         if (bpc < 1) bpc = 1;
         uint64_t g = (uint64_t)bpc * (uint64_t)c->num_cus;
         if (items > 0 && g > items) g = items;
@@ -336,6 +358,7 @@ int pass_from_bucket(ewfa_ctx* c, const ewfa::BucketPolicy& b, uint32_t blen, ui
         }
     }
     choose_off16(p, blen);
+    choose_band_cap(c, p);
     return finish_pass(c, p, items);
 }
 
@@ -571,7 +594,8 @@ int launch_pass(const ewfa::AlignArgs& a, const Pass& p, hipStream_t s) {
 int enqueue_bound(ewfa_ctx* c, ewfa::AlignArgs& a, const Pass& p, uint32_t* queue, uint32_t n, hipStream_t s) {
     a.bound = nullptr;
     const size_t lds = ewfa::lds_bytes_bound(p.seq_words);
-    if (p.reg || p.dcap < BOUND_MIN_DCAP || lds > 65536 || n == 0) return EWFA_OK;
+    // (a pass with narrow slots always gets its bounds, as it must read them)
+    if (p.reg || p.dcap < BOUND_MIN_DCAP || lds > 65536 || (n == 0 && !p.band_cap)) return EWFA_OK;
     ewfa::AlignArgs b = a;
     b.queue = queue;
     b.bound = scratch_of(c).bound;
@@ -604,6 +628,7 @@ void fill_pass_args(ewfa::AlignArgs& a, const Pass& p, ewfa_ctx* c) {
     a.bt_global = sx.bt + p.slot_off / 8;
     a.bt_slot_words = 2u * (uint64_t)ewfa::bt_chunks((uint32_t)p.dcap);
     a.off16 = p.off16;
+    a.band_cap = p.band_cap;
     a.gslot = p.gslot ? sx.gslot + p.gslot_off / 8 : nullptr;
     a.gslot_words = p.gslot ? p.gslot_bytes / 8 : 0;
 }

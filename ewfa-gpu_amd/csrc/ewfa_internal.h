@@ -91,6 +91,10 @@ struct AlignArgs {
     // bound_kernel and read by the block kernel, which then leaves out the
     // cells no path of that cost can use; null: no bounds
     uint32_t* bound;
+    // bounded passes with narrow slots: each offset generation holds the
+    // band_cap + 6 cells of the diagonals a bound of at most band_cap leaves
+    // (pairs with larger bounds are deferred); 0: generations of 2*dcap + 5
+    int32_t band_cap;
     // 16-bit offsets in LDS (the pass's 2 * (max_len + dcap) fits)
     int32_t off16;
     // per-block slots in HBM (sequence copies and both offset generations,
@@ -130,6 +134,13 @@ inline size_t lds_bytes(int seq_words, int dcap, int bt_lds, int off16 = 0) {
     b = (b + 15) & ~(size_t)15;
     if (bt_lds) b += 16u * (size_t)bt_chunks((uint32_t)dcap);
     return b;
+}
+
+// LDS bytes of a bounded block pass's narrow slot (band_cap above): both
+// sequences and two offset generations of band_cap + 6 32-bit cells.
+inline size_t lds_bytes_narrow(int seq_words, int band_cap) {
+    size_t b = 16u * (size_t)seq_words + 8u * (size_t)(band_cap + 6);
+    return (b + 15) & ~(size_t)15;
 }
 
 // Guard words (64-bit) of a banded slot before the pattern copy and after

@@ -657,12 +657,43 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
         return;
     }
     const int dcap = a.dcap;
+    const int tk = tlen - plen;  // target diagonal, target offset tlen
+    const int tka = tk < 0 ? -tk : tk;
+    const int cap = a.max_distance;
+    int limit = dcap;
+    if (cap > 0 && cap - 1 < limit) limit = cap - 1;
+    // With an upper bound U on the distance, cell (d, k) is computed only when
+    // d + |k - tk| <= U: a path through any other cell costs more than U to
+    // reach the target diagonal, and the cells kept read only cells kept, so
+    // the values the search and its traceback use are unchanged.
+    const uint32_t ub = BOUNDED ? a.bound[pidx] : BOUND_NONE;
+    // Narrow slots (a.band_cap > 0): the search keeps the cells within
+    // U = min(bound, band_cap, limit), which is exact for every pair whose
+    // distance is at most U (the argument above holds for any U of at least
+    // the distance); one whose distance is larger ends the search undone at
+    // level U and is deferred.  The cells kept span the diagonals
+    // kmin = -floor((U - tk)/2) .. kmax = floor((U + tk)/2), at most U + 1,
+    // and an offset generation holds those with two cells either side,
+    // diagonal k at index k - kmin + 2 of band_cap + 6.  A pair with
+    // |tk| > U is deferred before it is staged.
+    const bool narrow = BOUNDED && a.band_cap > 0;
+    const int U = narrow ? (int)min(min(ub, (uint32_t)a.band_cap), (uint32_t)limit) : (int)ub;
+    const bool pruned = narrow || (BOUNDED && ub <= (uint32_t)limit);
+    if (narrow && tka > U) {
+        if (gtid == 0) {
+            const uint32_t at = atomicAdd(a.defer_count, 1u);
+            a.defer_list[at] = pidx;
+            res->status = PAIR_DEFERRED;
+        }
+        return;
+    }
+    const int kmin = narrow ? -((U - tk) >> 1) : 0, kmax = narrow ? (U + tk) >> 1 : 0;
     uint64_t* P = slot;
     uint64_t* T = slot + sw;
     const uint32_t* P32 = reinterpret_cast<const uint32_t*>(P);
     const uint32_t* T32 = reinterpret_cast<const uint32_t*>(T);
-    Off* X = reinterpret_cast<Off*>(slot + 2 * sw) + (dcap + 2);
-    Off* Y = X + (2 * dcap + 5);
+    Off* X = reinterpret_cast<Off*>(slot + 2 * sw) + (narrow ? 2 - kmin : dcap + 2);
+    Off* Y = X + (narrow ? a.band_cap + 6 : 2 * dcap + 5);
     uint64_t* BT = BT_LDS ? slot + slot_bt_word(sw, dcap) : bt_g;
     // the HBM backtrace slot as a buffer: stores take their offset in a scalar
     // (slots of passes with 32-bit byte offsets; the HBM-slot passes store
@@ -678,9 +709,9 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
     for (int i = gtid; i < nwt + 2; i += 64 * G) T[i] = i < nwt ? __builtin_bitreverse64(gt[i]) : 0ull;
     group_sync<G>();
 
-    const int tk = tlen - plen;  // target diagonal, target offset tlen
-    const int tka = tk < 0 ? -tk : tk;
-    // offsets are kept doubled (none: -2), positions too, as eq2_le3 takes them
+    // offsets are kept doubled (none: -2), positions too, as eq2_le3 takes
+    // them (diagonals -2..2 lie inside a narrow slot: kmin <= 0 <= kmax, as
+    // U >= |tk|)
     if (gtid == 0) {
         X[0] = 2 * extend_le3(P32, T32, plen, tlen, 0, 0);
         X[-1] = -2;
@@ -690,16 +721,6 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
     }
     group_sync<G>();
 
-    const int cap = a.max_distance;
-    int limit = dcap;
-    if (cap > 0 && cap - 1 < limit) limit = cap - 1;
-    // With an upper bound U on the distance, cell (d, k) is computed only when
-    // d + |k - tk| <= U: a path through any other cell costs more than U to
-    // reach the target diagonal, and the cells kept read only cells kept, so
-    // the values the search and its traceback use are unchanged.
-    const uint32_t ub = BOUNDED ? a.bound[pidx] : BOUND_NONE;
-    const bool pruned = BOUNDED && ub <= (uint32_t)limit;
-    const int U = (int)ub;
     bool done = (tk == 0) && (X[0] == 2 * tlen);
     int d = 0;
     Off* cur = X;
@@ -719,10 +740,11 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
         } else {
             block_level<G, BT_PTR, false, Off>(d, 0, 2 * d, wave, lane, cur, nxt, P32, T32, plen, tlen, base, BT, BTs);
         }
-        // sentinels for the next level's reads at |k| = d+1, d+2
+        // sentinels for the next level's reads at |k| = d+1, d+2 (in a narrow
+        // slot only those inside it: the next level reads none outside)
         if (gtid < 4) {
             const int e = gtid < 2 ? -(d + 1 + gtid) : (d + 1 + (gtid - 2));
-            nxt[e] = -2;
+            if (!narrow || (e >= kmin - 2 && e <= kmax + 2)) nxt[e] = -2;
         }
         group_sync<G>();
         base += (uint32_t)((d >> 5) + 1);
@@ -732,7 +754,9 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
         nxt = tmp;
     }
 
-    finish_pair<false>(a, pidx, BT, gtid, done, d, limit, tk);
+    // (a narrow search that stopped undone below the level limit is deferred:
+    // the reference's cap applies only at that limit)
+    finish_pair<false>(a, pidx, BT, gtid, done, d, narrow ? U : limit, tk);
 }
 
 // ---- register wavefront (one wave per pair) --------------------------------
@@ -2026,6 +2050,7 @@ template <int G, bool BT_LDS>
 static hipError_t launch_align_t(const AlignArgs& a, bool persistent, uint32_t grid, size_t lds,
                                  hipStream_t stream) {
     if (a.bound && a.off16) return hipErrorInvalidValue;  // the bounded kernel keeps 32-bit offsets
+    if (a.band_cap > 0 && (!a.bound || a.off16 || a.gslot || BT_LDS)) return hipErrorInvalidValue;  // narrow slots need bounds
     if (a.gslot) {
         // slots in HBM: GSLOT_WAVES waves, 32-bit offsets, no dynamic LDS
         if (G != GSLOT_WAVES || BT_LDS || a.off16 || lds) return hipErrorInvalidValue;

@@ -19,7 +19,9 @@ const BucketPolicy& Policy::bucket_for(uint32_t len) const {
 Policy default_policy() {
     Policy p;
     // tuned on 500,000 pairs of log-uniform length 100..10,000 bp at 1..10%
-    // edits; longer pairs take a 16-wave block with as many levels as LDS holds
+    // edits; longer pairs take an 8-wave block with as many levels as LDS
+    // holds, bounded, in narrow slots (two blocks per CU: 20,000 pairs of
+    // 10..50 kbp in 211 ms, against 290 ms for one 16-wave block per CU)
     //              len_hi   waves dcap  bt_lds grab
     p.buckets.push_back({300u, 1, 63, 1, 16});
     p.buckets.push_back({700u, 1, 127, 1, 4});
@@ -27,7 +29,7 @@ Policy default_policy() {
     p.buckets.push_back({2500u, 2, 768, 0, 1});
     p.buckets.push_back({5000u, 2, 512, 0, 1});
     p.buckets.push_back({10500u, 2, 1024, 0, 1});
-    p.buckets.push_back({0xFFFFFFFFu, 16, 1 << 20, 0, 1});
+    p.buckets.push_back({0xFFFFFFFFu, 8, 1 << 20, 0, 1});
     return p;
 }
 

@@ -11,6 +11,10 @@ unbounded), low-complexity pairs full of equal-cost paths, each wave count
 of the block kernel (a level's first computed chunk can start inside a
 chunk, where lane 0 is outside the range), and distances above a bounded
 pass's budget (those pairs finish in the deferral passes, bounded again).
+Bucket passes keep only the diagonals within the bound in narrow slots
+(more blocks per CU); a pair whose bound exceeds the slot searches up to the
+slot's size and is deferred when its distance is larger, and a pair whose
+length difference exceeds it is deferred before it is staged.
 """
 import numpy as np
 import pytest
@@ -94,3 +98,28 @@ def test_distances_above_the_budget(tmp_path):
     with ewfa.Context(policy_path=policy_file(tmp_path, "block16", BOUNDED["block16"])) as ctx:
         res, cig = ctx.align_host(w.words, w.pairs, w.n, w.cigar_bytes)
     check(res, w.pairs, cig, d, st, "beyond-budget")
+
+
+def test_length_difference_beyond_a_narrow_slot(tmp_path):
+    # a budget of 2,048 levels gives the bucket pass narrow slots of 2,048
+    # diagonals; pairs whose lengths differ by more (the target diagonal lies
+    # outside every slot) are deferred unstaged and finish in the deferral
+    # passes, beside pairs that fit
+    rng = np.random.default_rng(44)
+    P, T = [], []
+    for i in range(12):
+        n = int(rng.integers(2500, 3500))
+        p = rng.integers(0, 4, n)
+        extra = int(rng.integers(2100, 2600)) if i % 2 == 0 else int(rng.integers(0, 300))
+        t = np.concatenate([p[: n // 2], rng.integers(0, 4, extra), p[n // 2:]])
+        if i % 4 in (1, 2):
+            p, t = t, p  # the pattern the longer one
+        P.append("".join("ACGT"[x] for x in p).encode())
+        T.append("".join("ACGT"[x] for x in t).encode())
+    print(f"case narrow slot, length differences to 2,600: {len(P)} pairs", flush=True)
+    d, st = oracle_streams(P, T)
+    assert (d > 2048).any() and (d <= 2048).any()
+    words, pairs, cb, ml = ewfa.from_sequences(P, T)
+    with ewfa.Context(policy_path=policy_file(tmp_path, "block8", "4000000000 8 2048 0 1\n")) as ctx:
+        res, cig = ctx.align_host(words, pairs, len(P), cb)
+    check(res, pairs, cig, d, st, "narrow-tk")
