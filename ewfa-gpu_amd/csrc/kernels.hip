@@ -45,6 +45,13 @@ __device__ __forceinline__ int writelane_i32(int x, int v, int lane) {
     return x;
 }
 
+// x with lane L (a constant) set to the uniform value v.
+template <int L>
+__device__ __forceinline__ int writelane_at(int x, int v) {
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(x) : "s"(v), "i"(L));
+    return x;
+}
+
 // plane with its bits moved up one place and bit 0 set, per lane, where
 // a >= b: a compare into VCC, then plane + plane + carry (v_addc), two VALU
 // instructions.  After n calls bit i holds the compare of the call n-1-i.
@@ -583,6 +590,19 @@ __device__ __forceinline__ void block_chunk(int c, int d, int ilo, int ihi, int 
             } while (e == 64);
         }
         nxt[(kc >> 1) + lane] = (Off)(val + acc);
+        if constexpr (FULL && !BT_PTR) {
+            // a whole chunk: lanes 0..3 store the four ballot dwords, and the
+            // other lanes' stores fall past the buffer's record count and are
+            // dropped, so no branch around a one-lane store is needed
+            int w = acc;  // (any register: lanes 4..63 are not stored)
+            w = writelane_at<0>(w, (int)(uint32_t)lo);
+            w = writelane_at<1>(w, (int)(uint32_t)(lo >> 32));
+            w = writelane_at<2>(w, (int)(uint32_t)hi);
+            w = writelane_at<3>(w, (int)(uint32_t)(hi >> 32));
+            const int vo = lane < 4 ? 4 * lane : 0x7FFFFFF0;
+            __builtin_amdgcn_raw_buffer_store_b32(w, BTs, vo, (int)(16u * (base + (uint32_t)c)), 0);
+            return;
+        }
         // the chunk's first lane in the range stores its ballots: lane 0,
         // unless a bounded range starts inside the chunk
         bool first;
