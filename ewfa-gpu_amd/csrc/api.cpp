@@ -97,7 +97,7 @@ struct ewfa_ctx {
     hipEvent_t ev_fork = nullptr, ev_join[MAX_FANOUT - 1] = {};
     int num_cus = 0;
     size_t lds_max = 0;
-    DevBuf ops, bt, ctrl, list_a, list_b, buckets, runs, list_fb, bound, gslot;
+    DevBuf ops, bt, ctrl, list_a, list_b, buckets, runs, list_fb, bound, gslot, order;
     uint32_t* host_ctrl = nullptr;  // pinned mirror of the control words
     float align_ms = 0.f, emit_ms = 0.f;
     bool timing = false;
@@ -526,6 +526,7 @@ ewfa::Scratch scratch_of(ewfa_ctx* c) {
     x.list_fb = c->list_fb.as<uint32_t>();
     x.buckets = c->buckets.as<uint32_t>();
     x.bound = c->bound.as<uint32_t>();
+    x.order = c->order.as<uint32_t>();
     x.bt = c->bt.as<uint64_t>();
     x.runs = c->runs.as<uint8_t>();
     x.gslot = c->gslot.as<uint64_t>();
@@ -573,6 +574,9 @@ int reserve_plan(ewfa_ctx* c, const Plan& pl, size_t n) {
     CHK(c->gslot.reserve(c, pl.gslot_bytes));
     if (pl.nb > 1) CHK(c->buckets.reserve(c, (size_t)pl.nb * std::max<size_t>(n, 1) * 4));
     CHK(c->bound.reserve(c, std::max<size_t>(n, 1) * 4));
+    bool narrow = false;
+    for (int b = 0; b < pl.nb; ++b) narrow = narrow || pl.pass[b].band_cap > 0;
+    if (narrow) CHK(c->order.reserve(c, (size_t)std::max(pl.nb, 1) * std::max<size_t>(n, 1) * 4));
     return EWFA_OK;
 }
 
@@ -821,6 +825,14 @@ int enqueue_plan(ewfa_ctx* c, const Plan& pl, const uint64_t* words, const ewfa_
                 a.fallback_count = ctrl + CTRL_FALLBACK;
             }
             CHK(enqueue_bound(c, a, p, ctrl + CTRL_BOUNDQ + b, n, q));
+            if (p.band_cap > 0 && a.bound && sx.order) {
+                // the pass takes its pairs by decreasing bound, share by share
+                uint32_t* ord = sx.order + (size_t)b * n;
+                int shift = 0;
+                while ((p.band_cap >> shift) >= ewfa::ORDER_BINS) ++shift;
+                HIPCHK(ewfa::launch_order_by_bound(a.list, a.count_ptr, a.count, a.bound, ord, shift, q));
+                a.list = ord;
+            }
             CHK(launch_pass(a, p, q));
             if (!p.reg) {
                 ewfa::EmitArgs e{};
@@ -1204,6 +1216,7 @@ int align_device_in(ewfa_ctx_t* c, const uint64_t* d_words, const ewfa_pair_t* d
     x.list_fb = reinterpret_cast<uint32_t*>(take(n * 4));
     x.bound = reinterpret_cast<uint32_t*>(take(n * 4));
     x.buckets = pl.nb > 1 ? reinterpret_cast<uint32_t*>(take((size_t)pl.nb * n * 4)) : nullptr;
+    x.order = nullptr;  // bounded passes keep their lists in order
     x.region = bytes - off;
     x.bt = reinterpret_cast<uint64_t*>(b + off);
     x.runs = reinterpret_cast<uint8_t*>(b + off);
@@ -1429,7 +1442,7 @@ int ewfa_destroy(ewfa_ctx_t* c) {
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     for (DevBuf* b : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->runs, &c->list_fb, &c->bound,
-                      &c->gslot, &c->h_words, &c->h_pairs, &c->h_results, &c->h_cigars, &c->dense, &c->scan})
+                      &c->gslot, &c->order, &c->h_words, &c->h_pairs, &c->h_results, &c->h_cigars, &c->dense, &c->scan})
         b->release();
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -1516,7 +1529,7 @@ int ewfa_context_memory(ewfa_ctx_t* c, size_t* bytes) {
     if (!c || !bytes) return fail(EWFA_ERR_ARG, "null argument");
     size_t b = 0;
     for (const DevBuf* d : {&c->ops, &c->bt, &c->ctrl, &c->list_a, &c->list_b, &c->buckets, &c->runs,
-                            &c->list_fb, &c->bound, &c->gslot, &c->h_words, &c->h_pairs, &c->h_results,
+                            &c->list_fb, &c->bound, &c->gslot, &c->order, &c->h_words, &c->h_pairs, &c->h_results,
                             &c->h_cigars, &c->dense, &c->scan})
         b += d->bytes;
     *bytes = b + c->retired_bytes;

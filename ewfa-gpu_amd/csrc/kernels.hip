@@ -1544,6 +1544,42 @@ __global__ __launch_bounds__(64) void bound_kernel(AlignArgs a) {
     }
 }
 
+// Block q reorders XCD share q of a list (items [count*q/8, count*(q+1)/8),
+// the share the pass hands out from counter q) by decreasing distance bound:
+// a counting sort over 64 bins of bound >> shift, no bound first.  A share's
+// longest searches then start first and its last ones are short, so the
+// pass's tail (the last searches running on few CUs) is short too.
+
+__global__ __launch_bounds__(1024) void order_by_bound_kernel(const uint32_t* list, const uint32_t* count_ptr,
+                                                              uint32_t count, const uint32_t* bound, uint32_t* out,
+                                                              int shift) {
+    __shared__ uint32_t at[ORDER_BINS];
+    const uint32_t n = count_ptr ? *count_ptr : count;
+    const uint32_t q = blockIdx.x;
+    const uint32_t lo = (uint32_t)(((uint64_t)n * q) / XCDS), hi = (uint32_t)(((uint64_t)n * (q + 1)) / XCDS);
+    const auto bin_of = [&](uint32_t pidx) -> uint32_t {
+        const uint32_t u = bound[pidx];
+        return u == BOUND_NONE ? 0u : (uint32_t)(ORDER_BINS - 1) - min(u >> shift, (uint32_t)(ORDER_BINS - 1));
+    };
+    if (threadIdx.x < ORDER_BINS) at[threadIdx.x] = 0u;
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&at[bin_of(list ? list[i] : i)], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = lo;  // bin b's items start at lo + (items of bins before b)
+        for (int b = 0; b < ORDER_BINS; ++b) {
+            const uint32_t c = at[b];
+            at[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const uint32_t pidx = list ? list[i] : i;
+        out[atomicAdd(&at[bin_of(pidx)], 1u)] = pidx;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // emit kernel: one lane per pair
 
@@ -2111,6 +2147,13 @@ hipError_t launch_align(const AlignArgs& a, int waves, bool persistent, uint32_t
                           : launch_align_t<16, false>(a, persistent, grid, lds, stream);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_order_by_bound(const uint32_t* list, const uint32_t* count_ptr, uint32_t count,
+                                 const uint32_t* bound, uint32_t* out, int shift, hipStream_t stream) {
+    hipLaunchKernelGGL(order_by_bound_kernel, dim3(XCDS), dim3(1024), 0, stream, list, count_ptr, count, bound, out,
+                       shift);
+    return hipGetLastError();
 }
 
 hipError_t launch_bound(const AlignArgs& a, uint32_t grid, size_t lds, hipStream_t stream) {

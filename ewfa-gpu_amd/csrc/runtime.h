@@ -24,6 +24,7 @@ struct Scratch {
     uint32_t* list_fb;
     uint32_t* buckets;   // per-bucket pair lists
     uint32_t* bound;     // per-pair distance bounds
+    uint32_t* order;     // bounded bucket lists by decreasing bound (null: kept in list order)
     uint64_t* bt;        // HBM backtrace slots of the block passes
     uint8_t* runs;       // run rows of the multi-chunk register passes
     uint64_t* gslot;     // HBM slots of the passes beyond what LDS holds
