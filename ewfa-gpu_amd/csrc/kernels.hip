@@ -544,20 +544,20 @@ __device__ __forceinline__ int extend_le3(const uint32_t* P, const uint32_t* T, 
 // ballots).
 // BT_PTR: the op ballots are stored through BT (LDS, or HBM with 64-bit
 // addresses), else through the buffer resource BTs with a 32-bit offset.
+// kc = 128c - 2d: twice the diagonal 64c + lane - d of the chunk's lanes,
+// less 2*lane (uniform; its uses below add it to uniform bases, so that each
+// is one per-lane add), and bi = base + c, the chunk's backtrace index; both
+// kept by the caller, a scalar add a chunk.
 template <bool FULL, bool BT_PTR, bool BOUNDED, typename Off>
-__device__ __forceinline__ void block_chunk(int c, int d, int ilo, int ihi, int lane, const Off* curm1, Off* nxt,
-                                            const uint32_t* P32, const uint32_t* T32, int plen, int tlen,
-                                            uint32_t base, uint64_t* BT, __amdgpu_buffer_rsrc_t BTs) {
-    // k2: twice the diagonal, 2*lane + kc (kc uniform); its uses below add
-    // the uniform part to uniform bases, so that each is one per-lane add
-    int kc;
+__device__ __forceinline__ void block_chunk(int kc, uint32_t bi, int d, int ilo, int ihi, int lane, const Off* curm1,
+                                            Off* nxt, const uint32_t* P32, const uint32_t* T32, int plen, int tlen,
+                                            uint64_t* BT, __amdgpu_buffer_rsrc_t BTs) {
     bool in;
     if constexpr (BOUNDED) {
-        const int t = (c << 6) + lane - ilo;
+        // flattened index 64c + lane (diagonal + d) inside ilo..ihi
+        const int t = (kc >> 1) + d + lane - ilo;
         in = FULL || (unsigned)t <= (unsigned)(ihi - ilo);
-        kc = 2 * ((c << 6) - d);
     } else {
-        kc = (c << 7) - 2 * d;  // twice the diagonal 64c + lane - d, less 2*lane
         in = FULL || 2 * lane + kc <= 2 * d;
     }
     if (in) {
@@ -581,13 +581,19 @@ __device__ __forceinline__ void block_chunk(int c, int d, int ilo, int ihi, int 
         // taken as exactly 32 matched, continued 32 at a time.
         const int mc = min(2 * plen + 2 * lane + kc, 2 * tlen);
         int acc = med3_i32(eq2_lez(P32, T32, v, val), mc - val, 0);
-        if (acc >= 32) {
-            acc = 32;
-            int e;
+        // (a uniform test and loop, the continuing lanes as the exec mask:
+        // most chunks have no lane to continue)
+        uint64_t cm = __builtin_amdgcn_ballot_w64(acc >= 32);
+        if (cm) {
+            acc = min(acc, 32);
+            int e = 64;
             do {
-                e = min(eq2_le3(P32, T32, v + acc, val + acc), mc - val - acc);
-                acc += e;
-            } while (e == 64);
+                if (__builtin_amdgcn_inverse_ballot_w64(cm)) {
+                    e = min(eq2_le3(P32, T32, v + acc, val + acc), mc - val - acc);
+                    acc += e;
+                }
+                cm &= __builtin_amdgcn_ballot_w64(e == 64);
+            } while (cm);
         }
         nxt[(kc >> 1) + lane] = (Off)(val + acc);
         if constexpr (FULL && !BT_PTR) {
@@ -600,7 +606,7 @@ __device__ __forceinline__ void block_chunk(int c, int d, int ilo, int ihi, int 
             w = writelane_at<2>(w, (int)(uint32_t)hi);
             w = writelane_at<3>(w, (int)(uint32_t)(hi >> 32));
             const int vo = lane < 4 ? 4 * lane : 0x7FFFFFF0;
-            __builtin_amdgcn_raw_buffer_store_b32(w, BTs, vo, (int)(16u * (base + (uint32_t)c)), 0);
+            __builtin_amdgcn_raw_buffer_store_b32(w, BTs, vo, (int)(16u * bi), 0);
             return;
         }
         // the chunk's first lane in the range stores its ballots: lane 0,
@@ -614,12 +620,12 @@ __device__ __forceinline__ void block_chunk(int c, int d, int ilo, int ihi, int 
         }
         if (first) {
             if constexpr (BT_PTR) {
-                uint64_t* bq = BT + 2 * (size_t)(base + (uint32_t)c);
+                uint64_t* bq = BT + 2 * (size_t)bi;
                 bq[0] = lo;
                 bq[1] = hi;
             } else {
                 const u32x4 w = {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
-                __builtin_amdgcn_raw_buffer_store_b128(w, BTs, 0, (int)(16u * (base + (uint32_t)c)), 0);
+                __builtin_amdgcn_raw_buffer_store_b128(w, BTs, 0, (int)(16u * bi), 0);
             }
         }
     }
@@ -640,17 +646,18 @@ __device__ __forceinline__ void block_level(int d, int ilo, int ihi, int wave, i
     // a chunk c with c0 < c < cend (or c0 itself when ilo starts it) is full
     const int cfull = (BOUNDED && (ilo & 63)) ? c0 + 1 : c0;
     int c = c0 + wave;
+    int kc = (c << 7) - 2 * d;
+    uint32_t bi = base + (uint32_t)c;
     if (BOUNDED && c < cfull) {
-        block_chunk<false, BT_PTR, BOUNDED, Off>(c, d, ilo, ihi, lane, curm1, nxt, P32, T32, plen, tlen, base, BT,
-                                                 BTs);
+        block_chunk<false, BT_PTR, BOUNDED, Off>(kc, bi, d, ilo, ihi, lane, curm1, nxt, P32, T32, plen, tlen, BT, BTs);
         c += G;
+        kc += 128 * G;
+        bi += G;
     }
-    for (; c < cend; c += G)
-        block_chunk<true, BT_PTR, BOUNDED, Off>(c, d, ilo, ihi, lane, curm1, nxt, P32, T32, plen, tlen, base, BT,
-                                                BTs);
+    for (; c < cend; c += G, kc += 128 * G, bi += G)
+        block_chunk<true, BT_PTR, BOUNDED, Off>(kc, bi, d, ilo, ihi, lane, curm1, nxt, P32, T32, plen, tlen, BT, BTs);
     if (c == cend)
-        block_chunk<false, BT_PTR, BOUNDED, Off>(c, d, ilo, ihi, lane, curm1, nxt, P32, T32, plen, tlen, base, BT,
-                                                 BTs);
+        block_chunk<false, BT_PTR, BOUNDED, Off>(kc, bi, d, ilo, ihi, lane, curm1, nxt, P32, T32, plen, tlen, BT, BTs);
 }
 
 // BOUNDED: the pass reads a distance bound per pair (a.bound) and leaves out
