@@ -714,7 +714,7 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
         }
         return;
     }
-    const int kmin = narrow ? -((U - tk) >> 1) : 0, kmax = narrow ? (U + tk) >> 1 : 0;
+    const int kmin = narrow ? -((U - tk) >> 1) : 0;
     uint64_t* P = slot;
     uint64_t* T = slot + sw;
     const uint32_t* P32 = reinterpret_cast<const uint32_t*>(P);
@@ -734,17 +734,31 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
     const uint64_t* gt = a.words + pr.text_off;
     for (int i = gtid; i < nwp + 2; i += 64 * G) P[i] = i < nwp ? __builtin_bitreverse64(gp[i]) : 0ull;
     for (int i = gtid; i < nwt + 2; i += 64 * G) T[i] = i < nwt ? __builtin_bitreverse64(gt[i]) : 0ull;
+    // offsets are kept doubled (none: -2), positions too, as eq2_le3 takes
+    // them.  An LDS slot's two generations start as none, in 8-byte stores
+    // (the slot's LDS is rounded up to 16 bytes): a cell is written only at
+    // levels at least |k|, so the cells a level reads past those computed
+    // before it (|k| = d+1, d+2 at level d+1) still hold none.  An HBM slot
+    // writes those sentinels a level at a time instead.
+    if constexpr (!GSLOT) {
+        const int cells = narrow ? 2 * (a.band_cap + 6) : 2 * (2 * dcap + 5);
+        const int words = (cells * (int)sizeof(Off) + 7) >> 3;
+        uint64_t* z = slot + 2 * sw;
+        const uint64_t none = sizeof(Off) == 2 ? 0xFFFEFFFEFFFEFFFEull : 0xFFFFFFFEFFFFFFFEull;
+        for (int i = gtid; i < words; i += 64 * G) z[i] = none;
+    }
     group_sync<G>();
 
-    // offsets are kept doubled (none: -2), positions too, as eq2_le3 takes
-    // them (diagonals -2..2 lie inside a narrow slot: kmin <= 0 <= kmax, as
+    // (diagonals -2..2 lie inside a narrow slot: kmin <= 0 <= kmax, as
     // U >= |tk|)
     if (gtid == 0) {
         X[0] = 2 * extend_le3(P32, T32, plen, tlen, 0, 0);
-        X[-1] = -2;
-        X[1] = -2;
-        X[-2] = -2;
-        X[2] = -2;
+        if constexpr (GSLOT) {
+            X[-1] = -2;
+            X[1] = -2;
+            X[-2] = -2;
+            X[2] = -2;
+        }
     }
     group_sync<G>();
 
@@ -767,11 +781,13 @@ __device__ void align_pair(const AlignArgs& a, uint32_t pidx, uint64_t* slot, ui
         } else {
             block_level<G, BT_PTR, false, Off>(d, 0, 2 * d, wave, lane, cur, nxt, P32, T32, plen, tlen, base, BT, BTs);
         }
-        // sentinels for the next level's reads at |k| = d+1, d+2 (in a narrow
-        // slot only those inside it: the next level reads none outside)
-        if (gtid < 4) {
-            const int e = gtid < 2 ? -(d + 1 + gtid) : (d + 1 + (gtid - 2));
-            if (!narrow || (e >= kmin - 2 && e <= kmax + 2)) nxt[e] = -2;
+        // an HBM slot's sentinels for the next level's reads at |k| = d+1,
+        // d+2 (an LDS slot holds none there from the start)
+        if constexpr (GSLOT) {
+            if (gtid < 4) {
+                const int e = gtid < 2 ? -(d + 1 + gtid) : (d + 1 + (gtid - 2));
+                nxt[e] = -2;
+            }
         }
         group_sync<G>();
         base += (uint32_t)((d >> 5) + 1);
