@@ -281,7 +281,7 @@ void choose_off16(Pass& p, uint32_t max_len) {
 void choose_band_cap(ewfa_ctx* c, Pass& p) {
     p.band_cap = 0;
     if (p.reg || p.gslot || p.bt_lds || p.off16 || p.dcap < BOUND_MIN_DCAP) return;
-    if (ewfa::lds_bytes_bound(p.seq_words) > 65536) return;  // no bounds for this pass
+    if (p.seq_words > ewfa::BOUND_MAX_SEQ_WORDS) return;  // no bounds for this pass
     const size_t budget = lds_budget(c);
     const size_t per_cu = budget / ewfa::lds_bytes(p.seq_words, p.dcap, 0, 0);
     const size_t room = budget / (per_cu + 1);
@@ -597,15 +597,15 @@ int launch_pass(const ewfa::AlignArgs& a, const Pass& p, hipStream_t s) {
 // 64 KiB of LDS go without.  `queue` must hold 0.
 int enqueue_bound(ewfa_ctx* c, ewfa::AlignArgs& a, const Pass& p, uint32_t* queue, uint32_t n, hipStream_t s) {
     a.bound = nullptr;
-    const size_t lds = ewfa::lds_bytes_bound(p.seq_words);
     // (a pass with narrow slots always gets its bounds, as it must read them)
-    if (p.reg || p.dcap < BOUND_MIN_DCAP || lds > 65536 || (n == 0 && !p.band_cap)) return EWFA_OK;
+    if (p.reg || p.dcap < BOUND_MIN_DCAP || p.seq_words > ewfa::BOUND_MAX_SEQ_WORDS || (n == 0 && !p.band_cap))
+        return EWFA_OK;
     ewfa::AlignArgs b = a;
     b.queue = queue;
     b.bound = scratch_of(c).bound;
-    const size_t per_cu = std::max<size_t>(1, std::min<size_t>(16, lds_budget(c) / lds));
-    const uint32_t grid = (uint32_t)std::min<size_t>((size_t)c->num_cus * per_cu, n);
-    HIPCHK(ewfa::launch_bound(b, grid, lds, s));
+    // one wave per block, 32 a CU (the search holds no LDS)
+    const uint32_t grid = (uint32_t)std::min<size_t>((size_t)c->num_cus * 32u, n);
+    HIPCHK(ewfa::launch_bound(b, grid, s));
     a.bound = b.bound;
     return EWFA_OK;
 }

@@ -187,16 +187,17 @@ constexpr uint32_t COMPACT_BLOCK = 1024;
 
 // Kernel launchers (kernels.hip).  All asynchronous on `stream`.
 // Distance bounds of list items from a 64-diagonal band: one wave per pair,
-// the sequences staged in `lds` bytes (lds_bytes_bound), a.queue as its
-// work counter.
-hipError_t launch_bound(const AlignArgs& a, uint32_t grid, size_t lds, hipStream_t stream);
+// reading the sequences in place, a.queue as its work counter.
+hipError_t launch_bound(const AlignArgs& a, uint32_t grid, hipStream_t stream);
 // Each XCD share of a bounded pass's list (list null: the identity; count
 // from *count_ptr when not null) into `out`, by decreasing distance bound
 // (ORDER_BINS bins of bound >> shift, none first).
 constexpr int ORDER_BINS = 64;
 hipError_t launch_order_by_bound(const uint32_t* list, const uint32_t* count_ptr, uint32_t count,
                                  const uint32_t* bound, uint32_t* out, int shift, hipStream_t stream);
-inline size_t lds_bytes_bound(int seq_words) { return 16u * (size_t)seq_words; }
+// Bounded passes: those whose sequence copies (16 bytes per word pair) take
+// at most 64 KiB, the block kernels' slots in LDS.
+constexpr int BOUND_MAX_SEQ_WORDS = 4096;
 hipError_t launch_align(const AlignArgs& a, int waves, bool persistent, uint32_t grid,
                         size_t lds, hipStream_t stream);
 hipError_t launch_align_reg(const AlignArgs& a, int nc, bool persistent, uint32_t grid, size_t lds,

@@ -151,6 +151,18 @@ __device__ __forceinline__ void stage_overlapped(uint64_t* S, const uint64_t* g,
     }
 }
 
+// A wave's next work item from a counter: lane 0 adds n, every lane gets the
+// old value.  The wave barrier joins the lanes' paths first: without it the
+// compiler may thread a branch on the lane elsewhere in the caller's loop
+// (`if (lane == 0) store`) past the add, into a loop of lanes 1..63 alone
+// that reads the counter's value from a lane that never adds.
+__device__ __forceinline__ uint32_t wave_take(uint32_t* counter, uint32_t n, int lane) {
+    __builtin_amdgcn_wave_barrier();
+    uint32_t b = 0;
+    if (lane == 0) b = atomicAdd(counter, n);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+}
+
 // Synchronisation of the threads working on one pair.
 template <int G>
 __device__ __forceinline__ void group_sync() {
@@ -1381,9 +1393,7 @@ __global__ __launch_bounds__(64, NC <= 3 ? 8 : 6) void align_reg_kernel(AlignArg
             const uint32_t lo = (uint32_t)(((uint64_t)count * q) / XCDS);
             const uint32_t hi = (uint32_t)(((uint64_t)count * (q + 1)) / XCDS);
             for (;;) {
-                uint32_t b = 0;
-                if (lane == 0) b = atomicAdd(a.xqueue + q * XQ_STRIDE, a.grab);
-                const uint32_t got = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+                const uint32_t got = wave_take(a.xqueue + q * XQ_STRIDE, a.grab, lane);
                 if (got >= hi - lo) break;
                 run(lo + got, min(lo + got + a.grab, hi));
             }
@@ -1391,9 +1401,7 @@ __global__ __launch_bounds__(64, NC <= 3 ? 8 : 6) void align_reg_kernel(AlignArg
         return;
     }
     for (;;) {
-        uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(a.queue, a.grab);
-        const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+        const uint32_t base = wave_take(a.queue, a.grab, lane);
         if (base >= count) break;
         run(base, min(base + a.grab, count));
     }
@@ -1473,22 +1481,48 @@ __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persiste
 constexpr int BAND_NONE_OFF = -(1 << 28);  // doubled offset of a diagonal with none
 constexpr int BAND_LOOK = 8;
 
-__device__ __forceinline__ uint32_t band_bound(const AlignArgs& a, uint32_t pidx, uint64_t* slot, int lane) {
+// A packed sequence read in place, for the bound search: the 32 bases from
+// a doubled position x2 (its first bit; first base in the high bits), the
+// position clamped into the sequence.  Each sequence is followed by a zero
+// pad word, so the word after the last base's can be read.  Past either end
+// the caller's count of bases left drops what such a window matches.
+struct PackedSeq {
+    const uint64_t* g;
+    int len2;  // twice the length
+    __device__ __forceinline__ uint64_t win(int x2) const {
+        if (len2 <= 0) return 0ull;
+        x2 = min(max(x2, 0), len2 - 2);
+        const int wi = x2 >> 6, sh = x2 & 63;
+        const uint64_t lo = g[wi], hi = g[wi + 1];
+        return sh ? (lo << sh) | (hi >> (64 - sh)) : lo;
+    }
+};
+
+__device__ __forceinline__ uint32_t band_bound(const AlignArgs& a, uint32_t pidx, int lane) {
     const ewfa_pair_t pr = a.pairs[pidx];
     const int plen = (int)pr.plen, tlen = (int)pr.tlen;
-    const int sw = a.seq_words;
-    if (pr.plen > a.max_len || pr.tlen > a.max_len || ((plen + 31) >> 5) + 2 > sw || ((tlen + 31) >> 5) + 2 > sw)
-        return BOUND_NONE;
-    uint64_t* P = slot;
-    uint64_t* T = slot + sw;
-    stage_plain(P, a.words + pr.pattern_off, plen, lane);
-    stage_plain(T, a.words + pr.text_off, tlen, lane);
-    group_sync<1>();
-    const uint32_t* P32 = reinterpret_cast<const uint32_t*>(P);
-    const uint32_t* T32 = reinterpret_cast<const uint32_t*>(T);
+    if (pr.plen > a.max_len || pr.tlen > a.max_len) return BOUND_NONE;
+    // the sequences are read in place (through the caches), not staged in
+    // LDS, so that the searches of many pairs run on a CU at once
+    const PackedSeq gP{a.words + pr.pattern_off, 2 * plen}, gT{a.words + pr.text_off, 2 * tlen};
+    // twice the equal leading bases of the 16- and 32-base windows at
+    // doubled positions (0..32, 0..64)
+    const auto eq16 = [&](int v2, int h2) -> int { return __clz((int)((gP.win(v2) ^ gT.win(h2)) >> 32)) & ~1; };
+    const auto eq32 = [&](int v2, int h2) -> int {
+        return __clzll((long long)(gP.win(v2) ^ gT.win(h2))) & ~1;
+    };
     const int tk = tlen - plen;
     const int tka = tk < 0 ? -tk : tk;
-    const int o00 = extend_le3(P32, T32, plen, tlen, 0, 0);
+    int o00;
+    {
+        const int most = 2 * min(plen, tlen);
+        int acc2 = 0, e;
+        do {
+            e = min(eq32(acc2, acc2), most - acc2);
+            acc2 += e;
+        } while (e == 64);
+        o00 = acc2 >> 1;
+    }
     if (tk == 0 && o00 == tlen) return 0u;
     int limit = a.dcap;
     const int cap = a.max_distance;
@@ -1527,11 +1561,11 @@ __device__ __forceinline__ uint32_t band_bound(const AlignArgs& a, uint32_t pidx
             // which nothing is matched
             const int v2 = val - k2;
             const int left = min(pl2 - v2, tl2 - val);
-            acc = med3_i32(eq2_le2(P32, T32, min(v2, pl2), min(val, tl2)), left, 0);
+            acc = med3_i32(eq16(min(v2, pl2), min(val, tl2)), left, 0);
             if (acc == 32) {
                 int e;
                 do {
-                    e = min(eq2_le3(P32, T32, v2 + acc, val + acc), left - acc);
+                    e = min(eq32(v2 + acc, val + acc), left - acc);
                     acc += e;
                 } while (e == 64);
             }
@@ -1551,19 +1585,15 @@ __device__ __forceinline__ uint32_t band_bound(const AlignArgs& a, uint32_t pidx
 // a.bound[pair] (BOUND_NONE when the band does not reach the end within the
 // pass's level limit).
 __global__ __launch_bounds__(64) void bound_kernel(AlignArgs a) {
-    extern __shared__ uint64_t lds[];
     const int lane = threadIdx.x;
     const uint32_t count = a.count_ptr ? *a.count_ptr : a.count;
     if (count == 0) return;  // as in align_reg_kernel
     for (;;) {
-        uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(a.queue, 1u);
-        const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+        const uint32_t item = wave_take(a.queue, 1u, lane);
         if (item >= count) break;
         const uint32_t pidx = a.list ? a.list[item] : item;
-        const uint32_t u = band_bound(a, pidx, lds, lane);
+        const uint32_t u = band_bound(a, pidx, lane);
         if (lane == 0) a.bound[pidx] = u;
-        group_sync<1>();  // the slot is staged again for the next pair
     }
 }
 
@@ -2179,9 +2209,9 @@ hipError_t launch_order_by_bound(const uint32_t* list, const uint32_t* count_ptr
     return hipGetLastError();
 }
 
-hipError_t launch_bound(const AlignArgs& a, uint32_t grid, size_t lds, hipStream_t stream) {
+hipError_t launch_bound(const AlignArgs& a, uint32_t grid, hipStream_t stream) {
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(bound_kernel, dim3(grid), dim3(64), lds, stream, a);
+    hipLaunchKernelGGL(bound_kernel, dim3(grid), dim3(64), 0, stream, a);
     return hipGetLastError();
 }
 
