@@ -21,7 +21,10 @@ Policy default_policy() {
     // tuned on 500,000 pairs of log-uniform length 100..10,000 bp at 1..10%
     // edits; longer pairs take an 8-wave block with as many levels as LDS
     // holds, bounded, in narrow slots (two blocks per CU: 20,000 pairs of
-    // 10..50 kbp in 211 ms, against 290 ms for one 16-wave block per CU)
+    // 10..50 kbp in 211 ms, against 290 ms for one 16-wave block per CU).
+    // Those are two buckets of the same parameters, split at 25 kbp: their
+    // passes run on separate streams, so that one's bound search (which
+    // holds no LDS) runs beside the other's block pass (189 -> 184 ms)
     //              len_hi   waves dcap  bt_lds grab
     p.buckets.push_back({300u, 1, 63, 1, 16});
     p.buckets.push_back({700u, 1, 127, 1, 4});
@@ -29,6 +32,7 @@ Policy default_policy() {
     p.buckets.push_back({2500u, 2, 768, 0, 1});
     p.buckets.push_back({5000u, 2, 512, 0, 1});
     p.buckets.push_back({10500u, 2, 1024, 0, 1});
+    p.buckets.push_back({25000u, 8, 1 << 20, 0, 1});
     p.buckets.push_back({0xFFFFFFFFu, 8, 1 << 20, 0, 1});
     return p;
 }

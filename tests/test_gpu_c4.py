@@ -1,9 +1,10 @@
 """Long reads (10-50 kbp at 10-15% edits, 30% of the edit budget as indel
 runs) through the default policy, bit for bit against the oracle.
 
-Pairs of this length take the built-in table's long-read bucket: a 16-wave
-block per pair with the backtrace in HBM, and per-pair distance bounds from
-the banded bound kernel that limit the cells the block computes.  Every
+Pairs of this length take the built-in table's two long-read buckets (up to
+25 kbp and beyond, run side by side): an 8-wave block per pair with the
+backtrace in HBM, and per-pair distance bounds from the banded bound kernel
+that limit the cells the block computes and size its narrow slot.  Every
 distance and stream must equal the oracle's (the reference's tie-break,
 src/kernels.cu:141-152, on every cell of the path), and distances the
 Levenshtein DP.
