@@ -271,8 +271,8 @@ void choose_off16(Pass& p, uint32_t max_len) {
     p.off16 = (!p.reg && !p.bt_lds && p.dcap < BOUND_MIN_DCAP && (size_t)max_len + (size_t)p.dcap <= 16383) ? 1 : 0;
 }
 
-// A bounded block pass (a budget of BOUND_MIN_DCAP levels or more, whose
-// sequences the bound kernel can stage) whose slot lets fewer blocks share a
+// A bounded block pass (a budget of BOUND_MIN_DCAP levels or more, sequences
+// of at most BOUND_MAX_SEQ_WORDS words) whose slot lets fewer blocks share a
 // CU than its narrow slot would: narrow slots sized for one block more per
 // CU.  Such a slot keeps only the diagonals within a pair's bound U, at most
 // U + 1 (against 2 * dcap + 5 cells a generation), and holds bounds up to
@@ -589,12 +589,12 @@ int launch_pass(const ewfa::AlignArgs& a, const Pass& p, hipStream_t s) {
 }
 
 // Ahead of a deep block pass: each of its pairs' distance bound from the
-// banded search (same list, level limit and staging), for the pass to leave
+// banded search (same list and level limit), for the pass to leave
 // out the cells no path within the bound uses.  The saving grows with the
 // distance while the bound's cost grows with the levels only: passes with
 // budgets under BOUND_MIN_DCAP levels (where the measured saving did not pay
-// for the bound), register passes, and sequences whose staging exceeds
-// 64 KiB of LDS go without.  `queue` must hold 0.
+// for the bound), register passes, and sequences of more than
+// BOUND_MAX_SEQ_WORDS words go without.  `queue` must hold 0.
 int enqueue_bound(ewfa_ctx* c, ewfa::AlignArgs& a, const Pass& p, uint32_t* queue, uint32_t n, hipStream_t s) {
     a.bound = nullptr;
     // (a pass with narrow slots always gets its bounds, as it must read them)

@@ -513,19 +513,8 @@ __device__ __forceinline__ int eq2_le3(const uint32_t* P, const uint32_t* T, int
 }
 
 // The same over a 16-base window (two words per sequence): twice the equal
-// leading bases, 0..32.
-__device__ __forceinline__ int eq2_le2(const uint32_t* P, const uint32_t* T, int v2, int h2) {
-    const uint32_t* p = P + (v2 >> 5);
-    const uint32_t* t = T + (h2 >> 5);
-    const uint32_t x = __builtin_amdgcn_alignbit(p[1], p[0], (uint32_t)v2) ^
-                       __builtin_amdgcn_alignbit(t[1], t[0], (uint32_t)h2);
-    const uint32_t z = x ? (uint32_t)__builtin_ctz(x) : 32u;
-    return (int)(z & ~1u);
-}
-
-// As eq2_le2 without its clamp: twice the equal leading bases (0..30) when
-// the 16-base windows differ, and 62 when they match (the first set bit of
-// zero is all ones).
+// leading bases (0..30) when the windows differ, and 62 when they match (the
+// first set bit of zero is all ones); callers clamp it.
 __device__ __forceinline__ int eq2_lez(const uint32_t* P, const uint32_t* T, int v2, int h2) {
     const uint32_t* p = P + (v2 >> 5);
     const uint32_t* t = T + (h2 >> 5);
