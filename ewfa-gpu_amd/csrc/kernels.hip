@@ -1472,18 +1472,20 @@ constexpr int BAND_LOOK = 8;
 
 // A packed sequence read in place, for the bound search: the 32 bases from
 // a doubled position x2 (its first bit; first base in the high bits), the
-// position clamped into the sequence.  Each sequence is followed by a zero
-// pad word, so the word after the last base's can be read.  Past either end
-// the caller's count of bases left drops what such a window matches.
+// position clamped into the sequence.  Each sequence of n words is followed
+// by a readable pad word (index n), so both words read stay within n + 1,
+// also for an empty sequence.  Past either end the caller's count of bases
+// left drops what such a window matches.  No branch, so that the loads of
+// the pattern's and the text's windows are in flight together.
 struct PackedSeq {
     const uint64_t* g;
     int len2;  // twice the length
+    int nw;    // words of bases
     __device__ __forceinline__ uint64_t win(int x2) const {
-        if (len2 <= 0) return 0ull;
-        x2 = min(max(x2, 0), len2 - 2);
+        x2 = min(max(x2, 0), max(len2 - 2, 0));
         const int wi = x2 >> 6, sh = x2 & 63;
-        const uint64_t lo = g[wi], hi = g[wi + 1];
-        return sh ? (lo << sh) | (hi >> (64 - sh)) : lo;
+        const uint64_t lo = g[wi], hi = g[min(wi + 1, nw)];
+        return (lo << sh) | ((hi >> 1) >> (63 - sh));  // a funnel shift, also for sh = 0
     }
 };
 
@@ -1493,7 +1495,8 @@ __device__ __forceinline__ uint32_t band_bound(const AlignArgs& a, uint32_t pidx
     if (pr.plen > a.max_len || pr.tlen > a.max_len) return BOUND_NONE;
     // the sequences are read in place (through the caches), not staged in
     // LDS, so that the searches of many pairs run on a CU at once
-    const PackedSeq gP{a.words + pr.pattern_off, 2 * plen}, gT{a.words + pr.text_off, 2 * tlen};
+    const PackedSeq gP{a.words + pr.pattern_off, 2 * plen, (plen + 31) >> 5};
+    const PackedSeq gT{a.words + pr.text_off, 2 * tlen, (tlen + 31) >> 5};
     // twice the equal leading bases of the 16- and 32-base windows at
     // doubled positions (0..32, 0..64)
     const auto eq16 = [&](int v2, int h2) -> int { return __clz((int)((gP.win(v2) ^ gT.win(h2)) >> 32)) & ~1; };
