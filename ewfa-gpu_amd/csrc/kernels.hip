@@ -1419,9 +1419,7 @@ __global__ __launch_bounds__(64 * G) void align_kernel(AlignArgs a, int persiste
     // `grab` items from a work counter, the value seen by the whole group
     const auto take = [&](uint32_t* counter) -> uint32_t {
         if constexpr (G == 1) {
-            uint32_t b = 0;
-            if (tid == 0) b = atomicAdd(counter, a.grab);
-            return (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+            return wave_take(counter, a.grab, tid);
         } else {
             if (tid == 0) s_base = atomicAdd(counter, a.grab);
             __syncthreads();
