@@ -488,9 +488,11 @@ int next_pass(ewfa_ctx* c, uint32_t max_len, int prev_dcap, uint32_t items, Pass
     p.waves = 4;
     const int most_global = max_dcap_for(p.seq_words, 0, lds_budget(c));
     int want = std::max(prev_dcap * 4, 256);
-    // no pair needs more than max(plen, tlen) levels
+    // no pair needs more than max(plen, tlen) levels; a budget whose next
+    // step would reach that goes all the way (one pass fewer in the chain,
+    // whose passes mostly find no pairs)
     const int dmax = std::min((int)max_len + 1, gslot_max_dcap());
-    if (want > dmax) want = dmax;
+    if (want > dmax || (int64_t)want * 4 >= dmax) want = dmax;
     if (most_global <= prev_dcap) {
         // LDS holds no larger wavefront: the block's slot goes to HBM
         if (want <= prev_dcap) return fail(EWFA_ERR_RANGE, "no larger wavefront budget available");
